@@ -1,0 +1,133 @@
+"""Benchmark: batched perft on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 4096 positions
+generated on the GPU from seed 2 by uniformly random legal playouts of 8-40
+plies from the initial position, perft(4) of every position in one batched
+call; u64 per-position leaf counts.  A step is one such batch.  Inputs are
+resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0, N=1).  --gpus N>1 exits: this build runs on one
+GPU; positions shard across GPUs with a host-side sum (no collective).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
+HBM_PEAK = 8.0e12
+
+
+def ops_per_position():
+    """Algorithmic 64-bit integer ops of one leaf-kernel position (apply +
+    analyse + count), counted once by the op-counting host build of the
+    device core (tests/hostcore/opcount.cpp) over this workload's leaf level."""
+    path = os.path.join(ROOT, "zeta_amd", "ops_per_position.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(pos, depth, threads):
+    from oracle import refcpu
+    sample = pos[:256]
+    t0 = time.perf_counter()
+    c = refcpu.perft_batch(sample, depth, threads)
+    dt = time.perf_counter() - t0
+    return {"value": float(c.sum()) / dt, "unit": "leaves/s", "cores": threads, "kind": "port",
+            "sample": "oracle perft(%d) of the first %d workload positions (%d leaves, %.1f s, %d threads)"
+                      % (depth, len(sample), int(c.sum()), dt, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--depth", type=int, default=4)
+    ap.add_argument("--positions", type=int, default=4096)
+    ap.add_argument("--nobulk", action="store_true", help="visit every leaf (no bulk counting)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (positions shard across GPUs with a "
+              "host-side sum; no multi-GPU run is part of it)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import zeta_amd
+
+    zeta_amd.init()
+    pos = zeta_amd.playouts(2, args.positions, 8, 40)
+    d_pos = torch.from_numpy(pos.view(np.uint8).copy()).cuda()
+    d_cnt = torch.zeros(args.positions, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        zeta_amd.perft_device(d_pos.data_ptr(), args.positions, args.depth, d_cnt.data_ptr(), nobulk=args.nobulk)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    counts = d_cnt.cpu().numpy().view(np.uint64).copy()
+    leaves = int(counts.sum())
+    zeta_amd.set_timing(True)
+    for _ in range(1):
+        step()
+    torch.cuda.synchronize()
+    kst = zeta_amd.stats()
+    zeta_amd.set_timing(False)
+    zeta_amd.stats_reset()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.steps
+    value = leaves * args.steps / dt
+
+    # roofline of the dominant kernel, from its HIP-event time (separate timed pass)
+    leaf = kst.get("perft_leaf", (0, 0.0, 0))
+    roof = None
+    ops = ops_per_position()
+    if leaf[0] and ops:
+        per_launch_items = leaf[2] / leaf[0]
+        avg_s = leaf[1] / leaf[0] / 1e3
+        key = "nobulk" if args.nobulk else "bulk"
+        lane_ops = per_launch_items * ops[key] * 2  # 64-bit op = 2 int32 lane-ops
+        achieved = lane_ops / avg_s / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK / 1e12, 2),
+                "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved * 1e12 / VALU_PEAK, 4),
+                "traffic": ops.get("traffic_bytes_per_launch"),
+                "kernel": "perft_leaf", "avg_launch_ms": round(leaf[1] / leaf[0], 4),
+                "items_per_launch": int(per_launch_items), "ops64_per_item": ops[key]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cpu = cpu_baseline(pos, args.depth, threads)
+
+    out = {"metric": "perft nodes/s", "value": value, "unit": "leaves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded random playouts, generated on GPU)",
+           "config": {"workload": "batched perft(%d), %d playout positions seed 2, 8-40 plies%s"
+                                  % (args.depth, args.positions, ", no bulk counting" if args.nobulk else ""),
+                      "positions": args.positions, "depth": args.depth, "leaves_per_step": leaves,
+                      "bulk_counting": not args.nobulk},
+           "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in kst.items() if v[0]},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

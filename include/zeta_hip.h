@@ -1,0 +1,104 @@
+/*
+ * zeta_hip.h -- C-ABI of libzetahip.so, the MI355X (gfx950) hot path of the
+ * Zeta chess engine: batched perft, batched fixed-depth alpha-beta search,
+ * learned move ordering and a continuous-batching search service.
+ *
+ * Plain C: pointers and sizes, no HIP or torch types.  Every function returns
+ * 0 on success or a negative ZETA_E* code; zeta_last_error() gives the text.
+ * One library context per process (the reference is single-threaded too:
+ * src/clrun.c keeps one device, one queue and process-global buffers).
+ *
+ * Positions cross the boundary as the reference's 7 x u64 quad-bitboard
+ * (src/types.h:60-66; src/zeta.h GLOBAL_BOARD) plus the side to move.
+ */
+#ifndef ZETA_HIP_H
+#define ZETA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZETA_OK 0
+#define ZETA_EINVAL (-1)   /* bad argument */
+#define ZETA_ENODEV (-2)   /* no usable GPU / HIP runtime error at init */
+#define ZETA_EHIP (-3)     /* HIP runtime error */
+#define ZETA_ENOMEM (-4)   /* device allocation failed or budget exceeded */
+#define ZETA_ESTATE (-5)   /* library not initialised / wrong state */
+#define ZETA_EBUSY (-6)    /* serving queue full */
+
+/* quad-bitboard slots (src/types.h:60-66) */
+#define ZETA_QBBBLACK 0 /* black pieces */
+#define ZETA_QBBP1 1    /* piece type bit 0 */
+#define ZETA_QBBP2 2    /* piece type bit 1 */
+#define ZETA_QBBP3 3    /* piece type bit 2 */
+#define ZETA_QBBPMVD 4  /* moved flags: castle rights, e.p. pawn */
+#define ZETA_QBBHASH 5  /* Zobrist hash (rotate-left scheme) */
+#define ZETA_QBBHMC 6   /* half-move clock */
+
+typedef struct {
+  uint64_t board[7];
+  int32_t stm; /* 0 white, 1 black */
+  int32_t pad;
+} zeta_pos; /* 64 bytes */
+
+typedef struct {
+  int device;               /* HIP device ordinal */
+  int tt_log2;              /* search TT entries = 2^tt_log2 (16 B each); 0 = no TT */
+  uint64_t frontier_budget; /* perft: max child moves held per level (0 = default 1<<28) */
+  int workers;              /* search: wavefront workers per position (ABDADA), 0 = default */
+  int verbose;
+} zeta_config;
+
+/* Library set-up / tear-down (replaces cl_init_device / cl_release_device,
+ * src/clrun.h:26,31). */
+int zeta_init(const zeta_config *cfg);
+int zeta_shutdown(void);
+int zeta_last_error(char *buf, size_t n);
+/* Name of the device in use (hipDeviceProp_t::gcnArchName + name). */
+int zeta_device_info(char *buf, size_t n);
+
+/* ------------------------------------------------------------------ perft
+ * Leaf counts of perft(depth) for n root positions, one u64 per position.
+ * Replaces cl_run_perft (src/clrun.h:29; src/clrun.c:979-1147) driven by
+ * perft() (src/search.c:225-271), which counts one root per launch.
+ * Node semantics: the reference's move set (queen-only promotion, its
+ * castling and e.p. rules); the count is the number of leaves (Q7).
+ * zeta_perft_batch: host buffers.  zeta_perft_batch_device: d_pos / d_counts
+ * are device pointers (n zeta_pos in, n u64 out); flags below. */
+#define ZETA_PERFT_NOBULK 1 /* visit every leaf (generate its moves) instead of counting at depth-1 */
+int zeta_perft_batch(const zeta_pos *pos, int n, int depth, uint64_t *counts);
+int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts, int flags);
+
+/* Seeded random-playout positions generated on the GPU: position i tries
+ * attempts a = 0,1,..; rng = splitmix64 seeded with
+ * seed ^ i*0x9E3779B97F4A7C15 ^ a*0xD1B54A32D192ED03; plies =
+ * minply + r % (maxply-minply+1); each ply plays legal move r % nlegal in
+ * canonical order (origin square ascending, then target ascending) with the
+ * host domove rules (src/bitboard.c:583-693); an attempt is rejected when a
+ * side has no legal move or the half-move clock reaches 100. */
+int zeta_playouts(uint64_t seed, int n, int minply, int maxply, zeta_pos *out);
+
+/* ------------------------------------------------------------------ stats */
+#define ZETA_MAX_KSTATS 16
+typedef struct {
+  char name[32];
+  uint64_t launches;
+  double ms;           /* summed HIP-event time of the kernel's launches */
+  uint64_t items;      /* units processed (positions, nodes, records) */
+} zeta_kstat;
+typedef struct {
+  int nk;
+  zeta_kstat k[ZETA_MAX_KSTATS];
+} zeta_stats;
+int zeta_stats_get(zeta_stats *st);
+int zeta_stats_reset(void);
+/* 1 = record HIP events around every kernel (needed for zeta_stats_get). */
+int zeta_set_timing(int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,27 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built libzetahip.so")
+
+
+@pytest.fixture(scope="session")
+def zeta():
+    """The product library, initialised on cuda:0 (GPU tests only)."""
+    import zeta_amd
+    zeta_amd.ensure_init()
+    return zeta_amd
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import refcpu
+    refcpu.lib()
+    return refcpu

@@ -1,0 +1,67 @@
+// Host build of the device chess core (zeta_amd/csrc/chess.h) for CPU unit
+// tests of the kernel logic.  Test-only: the product never loads this.
+#include "../../zeta_amd/csrc/chess.h"
+
+using namespace zc;
+
+static Pos load(const uint64_t *b, int stm) {
+  Pos p;
+  p.b0 = b[0]; p.b1 = b[1]; p.b2 = b[2]; p.b3 = b[3]; p.cr = b[4]; p.stm = (u32)stm;
+  return p;
+}
+
+extern "C" int hc_count(const uint64_t *b, int stm, int qs) {
+  Pos p = load(b, stm);
+  Legal L;
+  analyse(p, L);
+  return count_legal(p, L, qs != 0);
+}
+
+extern "C" int hc_enumerate(const uint64_t *b, int stm, int qs, uint32_t *moves) {
+  Pos p = load(b, stm);
+  Legal L;
+  analyse(p, L);
+  int n = 0;
+  u64 pcs = L.own;
+  while (pcs) {
+    int f = lsb(pcs);
+    pcs &= pcs - 1;
+    u64 t = piece_targets(p, L, f, qs != 0);
+    while (t) {
+      int to = lsb(t);
+      t &= t - 1;
+      moves[n++] = make_code(p, f, to);
+    }
+  }
+  return n;
+}
+
+extern "C" void hc_apply(const uint64_t *b, int stm, uint32_t m, int rook_flag, uint64_t *out) {
+  Pos p = load(b, stm);
+  Pos c = apply(p, m, rook_flag != 0);
+  out[0] = c.b0; out[1] = c.b1; out[2] = c.b2; out[3] = c.b3; out[4] = c.cr; out[5] = c.stm;
+}
+
+static uint64_t perft_rec(const Pos &p, int d) {
+  Legal L;
+  analyse(p, L);
+  if (d == 1) return (uint64_t)count_legal(p, L, false);
+  uint64_t s = 0;
+  u64 pcs = L.own;
+  while (pcs) {
+    int f = lsb(pcs);
+    pcs &= pcs - 1;
+    u64 t = piece_targets(p, L, f, false);
+    while (t) {
+      int to = lsb(t);
+      t &= t - 1;
+      s += perft_rec(apply(p, make_code(p, f, to), true), d - 1);
+    }
+  }
+  return s;
+}
+
+extern "C" uint64_t hc_perft(const uint64_t *b, int stm, int depth) {
+  if (depth == 0) return 1;
+  return perft_rec(load(b, stm), depth);
+}

@@ -1,0 +1,74 @@
+"""CPU tests of the device chess core (zeta_amd/csrc/chess.h) through a host
+build of the same source (tests/hostcore/chess_host.cpp): the set-wise move
+generator must produce exactly the oracle's move lists (codes included)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import refcpu as R
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "hostcore", "libchesshost.so")
+SRC = os.path.join(HERE, "hostcore", "chess_host.cpp")
+HDR = os.path.join(os.path.dirname(HERE), "zeta_amd", "csrc", "chess.h")
+
+
+@pytest.fixture(scope="module")
+def H():
+    if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(SRC), os.path.getmtime(HDR)):
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", SO, SRC])
+    L = ctypes.CDLL(SO)
+    vp = ctypes.c_void_p
+    L.hc_count.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hc_enumerate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.hc_perft.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.hc_perft.restype = ctypes.c_uint64
+    return L
+
+
+def _oracle_moves(p, qs):
+    _, out, _ = R.genmoves(p, qs)
+    b = np.ascontiguousarray(p["b"][0])
+    mv = []
+    for lid in range(64):
+        m = int(out[lid])
+        while m:
+            t = (m & -m).bit_length() - 1
+            m &= m - 1
+            mv.append(int(R.lib().rc_makemove_code(b.ctypes.data, int(p["stm"][0]), lid, t)))
+    return mv
+
+
+def test_movelists_match_oracle(H):
+    pos = np.concatenate([R.gen_playouts(2, 1500, 8, 40), R.gen_playouts(3, 500, 20, 60)])
+    buf = np.zeros(256, dtype=np.uint32)
+    for i in range(len(pos)):
+        p = pos[i:i + 1]
+        b = np.ascontiguousarray(p["b"][0])
+        for qs in (0, 1):
+            om = _oracle_moves(p, qs)
+            n = H.hc_enumerate(b.ctypes.data, int(p["stm"][0]), qs, buf.ctypes.data)
+            assert H.hc_count(b.ctypes.data, int(p["stm"][0]), qs) == len(om)
+            assert list(buf[:n]) == om  # same codes, same canonical order
+
+
+def test_perft_matches_oracle(H):
+    pos = R.gen_playouts(2, 120, 8, 40)
+    want = R.perft_batch(pos, 3)
+    for i in range(len(pos)):
+        b = np.ascontiguousarray(pos[i]["b"])
+        assert H.hc_perft(b.ctypes.data, int(pos[i]["stm"]), 3) == want[i]
+
+
+def test_special_positions(H):
+    fens = ["R6R/3Q4/1Q4Q1/4Q3/2Q4Q/Q4Q2/pp1Q4/kBNN1KB1 w - - 0 1",  # 218 moves
+            "r3k2r/8/8/8/8/8/8/R3K2R w KQkq - 0 1", "8/8/8/8/k2Pp2Q/8/8/3K4 b - - 0 1",
+            "8/8/8/2k5/3Pp3/8/8/4K2R b K - 0 1", "7k/5Q2/6K1/8/8/8/8/8 b - - 0 1"]
+    for fen in fens:
+        p = R.setboard(fen)
+        b = np.ascontiguousarray(p["b"][0])
+        for d in (1, 2, 3):
+            assert H.hc_perft(b.ctypes.data, int(p["stm"][0]), d) == R.perft(p, d)

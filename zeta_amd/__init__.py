@@ -1,0 +1,161 @@
+"""zeta_amd -- Python mirror of libzetahip.so, the MI355X (gfx950) hot path of
+the Zeta chess engine.
+
+Everything here goes through the C-ABI declared in include/zeta_hip.h (ctypes,
+no torch types).  There is no CPU fallback: if the HIP library is missing or
+no gfx950 device is present, the calls raise.
+
+Reference interface mirrored (shintakezou/Zeta 099m):
+  perft()        -> perft() + cl_run_perft  (src/search.c:225-271, src/clrun.c:979)
+  search()       -> rootsearch() + cl_run_alphabeta (src/search.c:35-223, src/clrun.c:675)
+  setboard()     -> setboard (src/io.c:335-532), via the product's own C parser
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzetahip.so")
+
+# zeta_pos: 7 x u64 quad-bitboard (src/types.h:60-66) + side to move
+POS_DTYPE = np.dtype([("b", "<u8", (7,)), ("stm", "<i4"), ("pad", "<i4")])
+assert POS_DTYPE.itemsize == 64
+
+ZETA_PERFT_NOBULK = 1
+
+_lib = None
+_up = False
+
+
+class ZetaError(RuntimeError):
+    pass
+
+
+class _KStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64),
+                ("ms", ctypes.c_double), ("items", ctypes.c_uint64)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("nk", ctypes.c_int), ("k", _KStat * 16)]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("tt_log2", ctypes.c_int),
+                ("frontier_budget", ctypes.c_uint64), ("workers", ctypes.c_int),
+                ("verbose", ctypes.c_int)]
+
+
+def lib():
+    """Load libzetahip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ZetaError("libzetahip.so not built (run __graft_entry__.build() or make -C zeta_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, u64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64
+        L.zeta_init.argtypes = [ctypes.POINTER(_Config)]
+        L.zeta_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.zeta_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.zeta_perft_batch.argtypes = [vp, i32, i32, vp]
+        L.zeta_perft_batch_device.argtypes = [vp, i32, i32, vp, i32]
+        L.zeta_playouts.argtypes = [u64, i32, i32, i32, vp]
+        L.zeta_stats_get.argtypes = [ctypes.POINTER(_Stats)]
+        L.zeta_set_timing.argtypes = [i32]
+        for f in ("zeta_init", "zeta_shutdown", "zeta_last_error", "zeta_device_info",
+                  "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
+                  "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().zeta_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ZetaError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def init(device=0, tt_log2=0, frontier_budget=0, workers=0, verbose=0):
+    """zeta_init: bind the gfx950 device (raises when there is none)."""
+    global _up
+    cfg = _Config(device, tt_log2, frontier_budget, workers, verbose)
+    _check(lib().zeta_init(ctypes.byref(cfg)), "zeta_init")
+    _up = True
+
+
+def ensure_init(**kw):
+    if not _up:
+        init(**kw)
+
+
+def shutdown():
+    global _up
+    if _lib is not None:
+        _check(_lib.zeta_shutdown(), "zeta_shutdown")
+    _up = False
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().zeta_device_info(buf, 256), "zeta_device_info")
+    return buf.value.decode()
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def as_positions(pos):
+    pos = np.ascontiguousarray(pos)
+    if pos.dtype != POS_DTYPE:
+        pos = pos.view(POS_DTYPE) if pos.dtype.itemsize == 64 else np.ascontiguousarray(pos, dtype=POS_DTYPE)
+    return pos
+
+
+def perft(pos, depth):
+    """Leaf counts of perft(depth) for every position (u64 array)."""
+    ensure_init()
+    pos = as_positions(pos)
+    out = np.zeros(len(pos), dtype=np.uint64)
+    _check(lib().zeta_perft_batch(_ptr(pos), len(pos), int(depth), _ptr(out)), "zeta_perft_batch")
+    return out
+
+
+def perft_device(d_pos, n, depth, d_counts, nobulk=False):
+    """zeta_perft_batch_device on device pointers (ints)."""
+    ensure_init()
+    _check(lib().zeta_perft_batch_device(ctypes.c_void_p(d_pos), int(n), int(depth), ctypes.c_void_p(d_counts),
+                                         ZETA_PERFT_NOBULK if nobulk else 0), "zeta_perft_batch_device")
+
+
+def playouts(seed, n, minply, maxply):
+    """Seeded random-playout positions, generated on the GPU."""
+    ensure_init()
+    out = np.zeros(n, dtype=POS_DTYPE)
+    _check(lib().zeta_playouts(int(seed), int(n), int(minply), int(maxply), _ptr(out)), "zeta_playouts")
+    return out
+
+
+def set_timing(on=True):
+    ensure_init()
+    _check(lib().zeta_set_timing(1 if on else 0), "zeta_set_timing")
+
+
+def stats():
+    """Per-kernel HIP-event totals: {name: (launches, ms, items)}."""
+    ensure_init()
+    st = _Stats()
+    _check(lib().zeta_stats_get(ctypes.byref(st)), "zeta_stats_get")
+    return {st.k[i].name.decode(): (st.k[i].launches, st.k[i].ms, st.k[i].items) for i in range(st.nk)}
+
+
+def stats_reset():
+    ensure_init()
+    _check(lib().zeta_stats_reset(), "zeta_stats_reset")

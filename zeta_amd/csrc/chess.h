@@ -1,0 +1,495 @@
+// chess.h -- device chess core for gfx950 (also compiles for the host so the
+// same source can be unit-tested without a GPU; the library itself only runs
+// it on the device).
+//
+// Positions use the reference's quad-bitboard (src/types.h:60-66):
+//   b0 = black pieces, b1..b3 = piece-type bit planes (type = b1 + 2*b2 + 4*b3:
+//   1 pawn, 2 knight, 3 king, 4 bishop, 5 rook, 6 queen), cr = "moved" flags
+//   (castle rights live on a1/e1/h1/a8/e8/h8, the e.p. pawn is the one cleared
+//   bit of ranks 4/5).
+//
+// The generator here is set-wise, not the reference's 64-lane square-wise one:
+// one lane analyses a whole position with Kogge-Stone occluded fills (opponent
+// attack map with the own king transparent, king rays for checkers and pins),
+// then counts legal moves direction by direction (every target of a
+// direction/offset has exactly one origin, so popcounts count moves).  It
+// generates exactly the reference's legal move set, including its quirks:
+// queen-only promotion (Q1, src/zetaperft.cl:1380-1384), the SMCRALL castle
+// precondition (Q2, src/zetaperft.cl:1254), e.p. only from the moved flags
+// with a full make/test legality check (src/zetaperft.cl:1222-1248), and
+// 6-bit-masked shift counts (Q4).
+#pragma once
+#include <stdint.h>
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ZD __host__ __device__ __forceinline__
+#define ZC_CONST __constant__
+#else
+// host-only build of this header (CPU unit tests of the device logic)
+#define ZD static inline
+#define ZC_CONST
+#endif
+
+namespace zc {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef int32_t s32;
+
+constexpr u64 NOTA = 0xFEFEFEFEFEFEFEFEULL;
+constexpr u64 NOTH = 0x7F7F7F7F7F7F7F7FULL;
+constexpr u64 FULL = ~0ULL;
+constexpr u64 RANKS45 = 0x000000FFFF000000ULL;
+constexpr u64 RANK4 = 0x00000000FF000000ULL;
+constexpr u64 RANK5 = 0x000000FF00000000ULL;
+constexpr u64 RANK3 = 0x0000000000FF0000ULL;
+constexpr u64 RANK6 = 0x0000FF0000000000ULL;
+constexpr u64 SMCRALL = 0x8900000000000091ULL;     // src/types.h:114 (d8, not e8)
+constexpr u64 SMCRWHITEQ = 0x0000000000000011ULL;  // src/types.h:119-124
+constexpr u64 SMCRWHITEK = 0x0000000000000090ULL;
+constexpr u64 SMCRBLACKQ = 0x1100000000000000ULL;
+constexpr u64 SMCRBLACKK = 0x9000000000000000ULL;
+
+enum { PNONE = 0, PAWN = 1, KNIGHT = 2, KING = 3, BISHOP = 4, ROOK = 5, QUEEN = 6 };
+
+ZD int popc(u64 x) { return __builtin_popcountll(x); }
+ZD int lsb(u64 x) { return __builtin_ctzll(x); }
+ZD int msb(u64 x) { return 63 - __builtin_clzll(x); }
+// Q4: shift counts are taken modulo 64, as OpenCL C does.
+ZD u64 bit(int sq) { return 1ULL << (sq & 63); }
+ZD u64 shlm(u64 x, int n) { return x << (n & 63); }
+ZD u64 shrm(u64 x, int n) { return x >> (n & 63); }
+ZD u64 sel(bool c, u64 x) { return x & (0ULL - (u64)c); }
+
+// ---- move code: the reference's 32-bit packing (src/types.h:67-74,126-141)
+ZD u32 mv_from(u32 m) { return m & 63; }
+ZD u32 mv_to(u32 m) { return (m >> 6) & 63; }
+ZD u32 mv_cpt(u32 m) { return (m >> 12) & 63; }
+ZD u32 mv_pfrom(u32 m) { return (m >> 18) & 15; }
+ZD u32 mv_pto(u32 m) { return (m >> 22) & 15; }
+ZD u32 mv_pcpt(u32 m) { return (m >> 26) & 15; }
+ZD u32 mv_make(u32 f, u32 t, u32 c, u32 pf, u32 pt, u32 pc) {
+  return f | (t << 6) | (c << 12) | (pf << 18) | (pt << 22) | (pc << 26);
+}
+constexpr u32 NULLMOVE = 0x41u;
+
+struct Pos {
+  u64 b0, b1, b2, b3, cr;
+  u32 stm;
+};
+
+ZD u32 piece_at(const Pos &p, int sq) {
+  return (u32)((shrm(p.b0, sq) & 1) | ((shrm(p.b1, sq) & 1) << 1) | ((shrm(p.b2, sq) & 1) << 2) |
+               ((shrm(p.b3, sq) & 1) << 3));
+}
+
+// ---- one-step shifts and Kogge-Stone occluded fills for the 8 directions
+// dir: 0 N, 1 S, 2 E, 3 W, 4 NE, 5 NW, 6 SE, 7 SW
+ZD u64 step(u64 x, int d) {
+  switch (d) {
+    case 0: return x << 8;
+    case 1: return x >> 8;
+    case 2: return (x << 1) & NOTA;
+    case 3: return (x >> 1) & NOTH;
+    case 4: return (x << 9) & NOTA;
+    case 5: return (x << 7) & NOTH;
+    case 6: return (x >> 7) & NOTA;
+    default: return (x >> 9) & NOTH;
+  }
+}
+// occluded fill: generators plus every square reachable through `empty`
+template <int D>
+ZD u64 ks_fill(u64 g, u64 e) {
+  constexpr int s = (D == 0 || D == 1) ? 8 : (D == 2 || D == 3) ? 1 : (D == 4 || D == 7) ? 9 : 7;
+  constexpr bool up = (D == 0 || D == 2 || D == 4 || D == 5);
+  constexpr u64 m = (D == 2 || D == 4 || D == 6) ? NOTA : (D == 3 || D == 5 || D == 7) ? NOTH : FULL;
+  e &= m;
+  if (up) {
+    g |= e & (g << s);
+    e &= e << s;
+    g |= e & (g << (2 * s));
+    e &= e << (2 * s);
+    g |= e & (g << (4 * s));
+  } else {
+    g |= e & (g >> s);
+    e &= e >> s;
+    g |= e & (g >> (2 * s));
+    e &= e >> (2 * s);
+    g |= e & (g >> (4 * s));
+  }
+  return g;
+}
+// squares a slider set attacks in direction D (through empties, incl. first blocker)
+template <int D>
+ZD u64 slide(u64 g, u64 empty) {
+  return step(ks_fill<D>(g, empty), D);
+}
+ZD u64 rook_att(u64 g, u64 empty) {
+  return slide<0>(g, empty) | slide<1>(g, empty) | slide<2>(g, empty) | slide<3>(g, empty);
+}
+ZD u64 bishop_att(u64 g, u64 empty) {
+  return slide<4>(g, empty) | slide<5>(g, empty) | slide<6>(g, empty) | slide<7>(g, empty);
+}
+ZD u64 knight_att(u64 n) {
+  u64 l1 = (n >> 1) & NOTH, l2 = (n >> 2) & 0x3F3F3F3F3F3F3F3FULL;
+  u64 r1 = (n << 1) & NOTA, r2 = (n << 2) & 0xFCFCFCFCFCFCFCFCULL;
+  u64 h1 = l1 | r1, h2 = l2 | r2;
+  return (h1 << 16) | (h1 >> 16) | (h2 << 8) | (h2 >> 8);
+}
+// the 8 knight offsets separately (each shift is injective on the board)
+ZD u64 knight_dir(u64 n, int k) {
+  switch (k) {
+    case 0: return (n << 17) & NOTA;
+    case 1: return (n << 15) & NOTH;
+    case 2: return (n << 10) & 0xFCFCFCFCFCFCFCFCULL;
+    case 3: return (n << 6) & 0x3F3F3F3F3F3F3F3FULL;
+    case 4: return (n >> 6) & 0xFCFCFCFCFCFCFCFCULL;
+    case 5: return (n >> 10) & 0x3F3F3F3F3F3F3F3FULL;
+    case 6: return (n >> 15) & NOTA;
+    default: return (n >> 17) & NOTH;
+  }
+}
+ZD u64 king_att(u64 k) {
+  u64 a = k | ((k << 1) & NOTA) | ((k >> 1) & NOTH);
+  a |= (a << 8) | (a >> 8);
+  return a ^ k;
+}
+ZD u64 pawn_att(u64 p, u32 color) {
+  return color ? (((p >> 7) & NOTA) | ((p >> 9) & NOTH)) : (((p << 9) & NOTA) | ((p << 7) & NOTH));
+}
+// full board lines through square k
+ZD u64 rank_line(int k) { return 0xFFULL << (k & 56); }
+ZD u64 file_line(int k) { return 0x0101010101010101ULL << (k & 7); }
+ZD u64 diag_line(int k) {  // a1-h8 direction
+  int d = (k & 7) - (k >> 3);
+  return d >= 0 ? (0x8040201008040201ULL >> (8 * d)) : (0x8040201008040201ULL << (-8 * d));
+}
+ZD u64 anti_line(int k) {  // h1-a8 direction
+  int s = (k & 7) + (k >> 3);
+  return s <= 7 ? (0x0102040810204080ULL >> (8 * (7 - s))) : (0x0102040810204080ULL << (8 * (s - 7)));
+}
+
+// ---- per-position legality context
+struct Legal {
+  u64 own, opp, occ, kbb;
+  u64 pawns, knights, kings, rq, bq;
+  u64 A;         // opponent attacks, own king transparent
+  u64 checkers;  // opponent pieces giving check
+  u64 cmask;     // allowed targets of non-king moves: ~0, block/capture set, or 0 (double check)
+  u64 pinR, pinF, pinD, pinA;  // own pieces pinned on the king's rank, file, diagonal, anti-diagonal
+  int ksq;
+  u32 stm;
+};
+
+template <int D>
+ZD void king_ray(const Legal &L, u64 sl, u64 &checkers, u64 &chkray, u64 &pin) {
+  u64 empty = ~L.occ;
+  u64 ray = slide<D>(L.kbb, empty);
+  u64 hit = ray & L.occ;
+  bool chk = (hit & sl) != 0;
+  checkers |= sel(chk, hit);
+  chkray |= sel(chk, ray);
+  u64 oh = hit & L.own;
+  u64 hit2 = slide<D>(oh, empty) & L.occ;
+  pin |= sel((hit2 & sl) != 0, oh);
+}
+
+ZD void analyse(const Pos &p, Legal &L) {
+  const u64 occ = p.b1 | p.b2 | p.b3;
+  const u64 white = p.b0 ^ occ;
+  L.stm = p.stm;
+  L.occ = occ;
+  L.own = p.stm ? p.b0 : white;
+  L.opp = p.stm ? white : p.b0;
+  L.pawns = p.b1 & ~p.b2 & ~p.b3;
+  L.knights = ~p.b1 & p.b2 & ~p.b3;
+  L.kings = p.b1 & p.b2 & ~p.b3;
+  L.rq = p.b3 & (p.b1 ^ p.b2);  // rook 101, queen 110
+  L.bq = p.b3 & ~p.b1;          // bishop 100, queen 110
+  L.kbb = L.own & L.kings;
+  L.ksq = L.kbb ? lsb(L.kbb) : 0;
+  const u64 orq = L.opp & L.rq, obq = L.opp & L.bq;
+  const u64 enk = ~(occ ^ L.kbb);
+  u64 A = pawn_att(L.opp & L.pawns, p.stm ^ 1u) | knight_att(L.opp & L.knights) | king_att(L.opp & L.kings);
+  A |= slide<0>(orq, enk) | slide<1>(orq, enk) | slide<2>(orq, enk) | slide<3>(orq, enk);
+  A |= slide<4>(obq, enk) | slide<5>(obq, enk) | slide<6>(obq, enk) | slide<7>(obq, enk);
+  L.A = A;
+  u64 checkers = (pawn_att(L.kbb, p.stm) & L.opp & L.pawns) | (knight_att(L.kbb) & L.opp & L.knights);
+  u64 chkray = 0, pr = 0, pf = 0, pd = 0, pa = 0;
+  king_ray<0>(L, orq, checkers, chkray, pf);
+  king_ray<1>(L, orq, checkers, chkray, pf);
+  king_ray<2>(L, orq, checkers, chkray, pr);
+  king_ray<3>(L, orq, checkers, chkray, pr);
+  king_ray<4>(L, obq, checkers, chkray, pd);
+  king_ray<7>(L, obq, checkers, chkray, pd);
+  king_ray<5>(L, obq, checkers, chkray, pa);
+  king_ray<6>(L, obq, checkers, chkray, pa);
+  L.checkers = checkers;
+  L.pinR = pr;
+  L.pinF = pf;
+  L.pinD = pd;
+  L.pinA = pa;
+  const int n = popc(checkers);
+  L.cmask = n == 0 ? FULL : (n == 1 ? (checkers | chkray) : 0ULL);
+}
+
+// is the side to move's king attacked after the e.p. capture from->to taking sqep?
+ZD bool ep_legal(const Legal &L, int from, int to, int sqep) {
+  const u64 occ2 = (L.occ ^ bit(from) ^ bit(sqep)) | bit(to);
+  const u64 opp2 = L.opp & ~bit(sqep);
+  const u64 e = ~occ2;
+  u64 att = (rook_att(L.kbb, e) & opp2 & L.rq) | (bishop_att(L.kbb, e) & opp2 & L.bq);
+  att |= knight_att(L.kbb) & opp2 & L.knights;
+  att |= pawn_att(L.kbb, L.stm) & opp2 & L.pawns;
+  att |= king_att(L.kbb) & opp2 & L.kings;
+  return att == 0;
+}
+
+// e.p.: the pawn that just double-pushed is the lowest cleared bit of ranks 4/5
+// (src/zetaperft.cl:1224-1230); returns the own pawns that may capture it.
+ZD u64 ep_candidates(const Pos &p, const Legal &L, int &sqep) {
+  const u64 w = ~p.cr & RANKS45;
+  sqep = w ? lsb(w) : 0;
+  u64 c = L.own & L.pawns & (p.stm ? RANK4 : RANK5);
+  return c & (bit(sqep + 1) | bit(sqep - 1));
+}
+
+// castling targets of the king (src/zetaperft.cl:1249-1297)
+ZD u64 castle_targets(const Pos &p, const Legal &L) {
+  const u64 cr = p.cr;
+  const int k = L.ksq;
+  if (!(cr & SMCRALL)) return 0;
+  const bool okq = p.stm ? ((~cr & SMCRBLACKQ) == SMCRBLACKQ) : ((~cr & SMCRWHITEQ) == SMCRWHITEQ);
+  const bool okk = p.stm ? ((~cr & SMCRBLACKK) == SMCRBLACKK) : ((~cr & SMCRWHITEK) == SMCRWHITEK);
+  const u64 ownrooks = L.own & L.rq & ~p.b2;  // rook = 101 (queen has b2 set)
+  u64 t = 0;
+  t |= sel(okq && (ownrooks & bit(k - 4)) && !(L.occ & (bit(k - 1) | bit(k - 2) | bit(k - 3))) &&
+               !(L.A & (bit(k) | bit(k - 1) | bit(k - 2))),
+           bit(k - 2));
+  t |= sel(okk && (ownrooks & bit(k + 3)) && !(L.occ & (bit(k + 1) | bit(k + 2))) &&
+               !(L.A & (bit(k) | bit(k + 1) | bit(k + 2))),
+           bit(k + 2));
+  return t;
+}
+
+// Number of legal moves (qs: captures and e.p. only, as the kernels' quiescence).
+ZD int count_legal(const Pos &p, const Legal &L, bool qs) {
+  const u64 own = L.own, occ = L.occ;
+  const u64 T = (qs ? L.opp : ~own) & L.cmask;
+  const u64 pinned = L.pinR | L.pinF | L.pinD | L.pinA;
+  int cnt = popc(king_att(L.kbb) & (qs ? L.opp : ~own) & ~L.A);
+  const u64 kn = own & L.knights & ~pinned;
+#pragma unroll
+  for (int k = 0; k < 8; k++) cnt += popc(knight_dir(kn, k) & T);
+  const u64 e = ~occ;
+  const u64 orq = own & L.rq, obq = own & L.bq;
+  cnt += popc(slide<0>(orq & (~pinned | L.pinF), e) & T);
+  cnt += popc(slide<1>(orq & (~pinned | L.pinF), e) & T);
+  cnt += popc(slide<2>(orq & (~pinned | L.pinR), e) & T);
+  cnt += popc(slide<3>(orq & (~pinned | L.pinR), e) & T);
+  cnt += popc(slide<4>(obq & (~pinned | L.pinD), e) & T);
+  cnt += popc(slide<7>(obq & (~pinned | L.pinD), e) & T);
+  cnt += popc(slide<5>(obq & (~pinned | L.pinA), e) & T);
+  cnt += popc(slide<6>(obq & (~pinned | L.pinA), e) & T);
+  const u64 P = own & L.pawns;
+  const u64 capD = P & (~pinned | L.pinD), capA = P & (~pinned | L.pinA);
+  const u64 tgt = L.opp & L.cmask;
+  if (p.stm) {
+    cnt += popc(((capD >> 9) & NOTH) & tgt) + popc(((capA >> 7) & NOTA) & tgt);
+  } else {
+    cnt += popc(((capD << 9) & NOTA) & tgt) + popc(((capA << 7) & NOTH) & tgt);
+  }
+  if (!qs) {
+    const u64 pp = P & (~pinned | L.pinF);
+    u64 s1, s2;
+    if (p.stm) {
+      s1 = (pp >> 8) & e;
+      s2 = ((s1 & RANK6) >> 8) & e;
+    } else {
+      s1 = (pp << 8) & e;
+      s2 = ((s1 & RANK3) << 8) & e;
+    }
+    cnt += popc(s1 & L.cmask) + popc(s2 & L.cmask);
+    cnt += popc(castle_targets(p, L));
+  }
+  int sqep;
+  u64 cand = ep_candidates(p, L, sqep);
+  while (cand) {
+    const int f = lsb(cand);
+    cand &= cand - 1;
+    const int to = (p.stm ? sqep - 8 : sqep + 8) & 63;
+    cnt += ep_legal(L, f, to, sqep) ? 1 : 0;
+  }
+  return cnt;
+}
+
+// Legal targets of the own piece on `sq` (used to enumerate moves in
+// canonical order: origin ascending, then target ascending).
+ZD u64 piece_targets(const Pos &p, const Legal &L, int sq, bool qs) {
+  const u64 b = bit(sq);
+  const u64 own = L.own, occ = L.occ;
+  const u32 type = (u32)((shrm(p.b1, sq) & 1) | ((shrm(p.b2, sq) & 1) << 1) | ((shrm(p.b3, sq) & 1) << 2));
+  if (type == KING) {
+    u64 t = king_att(b) & (qs ? L.opp : ~own) & ~L.A;
+    if (!qs) t |= castle_targets(p, L);
+    return t;
+  }
+  u64 pinmask = FULL;
+  if (L.pinR & b) pinmask = rank_line(L.ksq);
+  if (L.pinF & b) pinmask = file_line(L.ksq);
+  if (L.pinD & b) pinmask = diag_line(L.ksq);
+  if (L.pinA & b) pinmask = anti_line(L.ksq);
+  const u64 T = (qs ? L.opp : ~own) & L.cmask & pinmask;
+  const u64 e = ~occ;
+  u64 t = 0;
+  switch (type) {
+    case PAWN: {
+      t = pawn_att(b, p.stm) & L.opp & L.cmask & pinmask;
+      if (!qs) {
+        u64 s1 = (p.stm ? (b >> 8) : (b << 8)) & e;
+        u64 s2 = (p.stm ? ((s1 & RANK6) >> 8) : ((s1 & RANK3) << 8)) & e;
+        t |= (s1 | s2) & L.cmask & pinmask;
+      }
+      int sqep;
+      const u64 cand = ep_candidates(p, L, sqep);
+      if (cand & b) {
+        const int to = (p.stm ? sqep - 8 : sqep + 8) & 63;
+        if (ep_legal(L, sq, to, sqep)) t |= bit(to);
+      }
+      return t;
+    }
+    case KNIGHT: return knight_att(b) & T;
+    case BISHOP: return bishop_att(b, e) & T;
+    case ROOK: return rook_att(b, e) & T;
+    case QUEEN: return (rook_att(b, e) | bishop_att(b, e)) & T;
+    default: return 0;
+  }
+}
+
+// Pack from->to with the reference movepicker rules (src/zetaperft.cl:1363-1386):
+// e.p. capture square, captured piece, queen-only promotion (Q1).
+ZD u32 make_code(const Pos &p, int from, int to) {
+  const u32 pfrom = piece_at(p, from);
+  int sqcpt = to;
+  u32 pcpt = piece_at(p, to);
+  const bool pawn = ((pfrom >> 1) & 7) == PAWN;
+  if (pawn && p.stm && from - to != 8 && from - to != 16 && pcpt == PNONE) sqcpt = to + 8;
+  if (pawn && !p.stm && to - from != 8 && to - from != 16 && pcpt == PNONE) sqcpt = to - 8;
+  pcpt = piece_at(p, sqcpt);
+  const int rrank = p.stm ? ((to >> 3) ^ 7) : (to >> 3);
+  const u32 pto = (pawn && rrank == 7) ? ((QUEEN << 1) | (pfrom & 1)) : pfrom;
+  return mv_make((u32)from, (u32)to, (u32)sqcpt, pfrom, pto, pcpt);
+}
+
+ZD void put_piece(Pos &c, int sq, u32 pc) {
+  c.b0 |= shlm((u64)(pc & 1), sq);
+  c.b1 |= shlm((u64)((pc >> 1) & 1), sq);
+  c.b2 |= shlm((u64)((pc >> 2) & 1), sq);
+  c.b3 |= shlm((u64)((pc >> 3) & 1), sq);
+}
+
+// Apply a move: pieces as the kernels' domove (src/zetaperft.cl:413-484), then
+// the moved flags.  rook_flag selects the perft/host variant that also flags
+// the castle rook's origin (src/zetaperft.cl:1452-1455) or the alpha-beta
+// variant that does not (src/zeta.cl:2333-2345, quirk Q6).
+ZD Pos apply(const Pos &p, u32 m, bool rook_flag) {
+  Pos c = p;
+  const int f = (int)mv_from(m), t = (int)mv_to(m), x = (int)mv_cpt(m);
+  const u32 pf = mv_pfrom(m);
+  const bool king = ((pf >> 1) & 7) == KING;
+  const bool qside = king && f - t == 2, kside = king && t - f == 2;
+  u64 keep = ~bit(f) & ~bit(x) & ~bit(t);
+  if (qside) keep &= ~bit(f - 4);
+  if (kside) keep &= ~bit(f + 3);
+  c.b0 &= keep;
+  c.b1 &= keep;
+  c.b2 &= keep;
+  c.b3 &= keep;
+  put_piece(c, t, mv_pto(m));
+  const u32 rook = (ROOK << 1) | (pf & 1);
+  if (qside) put_piece(c, t + 1, rook);
+  if (kside) put_piece(c, t - 1, rook);
+  u64 cr = p.cr | bit(f) | bit(t) | bit(x);
+  if (rook_flag && qside) cr |= bit(f - 4);
+  if (rook_flag && kside) cr |= bit(f + 3);
+  const u32 col = pf & 1;
+  const int rf = col ? ((f >> 3) ^ 7) : (f >> 3), rt = col ? ((t >> 3) ^ 7) : (t >> 3);
+  const bool dbl = ((pf >> 1) & 7) == PAWN && rt - rf == 2;
+  cr |= RANKS45;
+  if (dbl) cr &= ~bit(t);
+  c.cr = cr;
+  c.stm = p.stm ^ 1u;
+  return c;
+}
+
+// ---- Zobrist hashing, rotate-left scheme: key(piece) rotated left by the
+// square (src/bitboard.c:461-508, src/zeta.cl:2297-2359).  12 piece keys
+// (colour*6 + type-1), 4 castle keys, 1 e.p. key rotated by the file; the
+// values are the reference's (src/zobrist.h:28-35) so hashes crossing the
+// boundary stay comparable.
+static ZC_CONST const u64 ZOBRIST[18] = {
+    0x9D39247E33776D41ULL, 0x2AF7398005AAA5C7ULL, 0x44DB015024623547ULL, 0x9C15F73E62A76AE2ULL,
+    0x75834465489C0C89ULL, 0x3290AC3A203001BFULL, 0x0FBBAD1F61042279ULL, 0xE83A908FF2FB60CAULL,
+    0x0D7E765D58755C10ULL, 0x1A083822CEAFE02DULL, 0x9605D5F0E25EC3B0ULL, 0xD021FF5CD13A2ED5ULL,
+    0x40BDF15D4A672E32ULL, 0x011355146FD56395ULL, 0x5DB4832046F3D9E5ULL, 0x239F8B2D7FF719CCULL,
+    0x05D1A1AE85B49AA1ULL, 0x679F848F6E8FC971ULL};
+
+ZD u64 rotl(u64 z, int s) { return shlm(z, s) | shrm(z, 64 - s); }
+ZD u64 piece_key(u32 pc, int sq) { return rotl(ZOBRIST[(pc & 1) * 6 + ((pc >> 1) & 7) - 1], sq); }
+ZD u64 hash_pieces(const Pos &p) {
+  u64 h = 0;
+  u64 occ = p.b1 | p.b2 | p.b3;
+  while (occ) {
+    int sq = lsb(occ);
+    occ &= occ - 1;
+    h ^= piece_key(piece_at(p, sq), sq);
+  }
+  return h;
+}
+ZD u64 hash_castle(u64 cr) {
+  u64 h = 0;
+  h ^= sel((~cr & SMCRWHITEQ) != 0, ZOBRIST[12]);
+  h ^= sel((~cr & SMCRWHITEK) != 0, ZOBRIST[13]);
+  h ^= sel((~cr & SMCRBLACKQ) != 0, ZOBRIST[14]);
+  h ^= sel((~cr & SMCRBLACKK) != 0, ZOBRIST[15]);
+  return h;
+}
+// host computehash (src/bitboard.c:461-508)
+ZD u64 hash_full(const Pos &p) {
+  u64 h = hash_pieces(p) ^ hash_castle(p.cr);
+  u64 w = ~p.cr & RANKS45;
+  int sqep = w ? lsb(w) : 0;
+  if (sqep) h ^= rotl(ZOBRIST[16], sqep & 7);
+  if (p.stm) h ^= 1ULL;
+  return h;
+}
+
+// k-th legal move in canonical order (origin ascending, then target ascending)
+ZD u32 nth_legal(const Pos &p, const Legal &L, int k) {
+  u64 pcs = L.own;
+  while (pcs) {
+    const int sq = lsb(pcs);
+    pcs &= pcs - 1;
+    u64 t = piece_targets(p, L, sq, false);
+    const int c = popc(t);
+    if (k < c) {
+      while (k-- > 0) t &= t - 1;
+      return make_code(p, sq, lsb(t));
+    }
+    k -= c;
+  }
+  return 0;
+}
+
+// host domove half-move clock rule (src/bitboard.c:604-689): +1, reset by
+// pawn moves, captures and castling
+ZD u32 next_hmc(u32 hmc, u32 m) {
+  const u32 pf = mv_pfrom(m);
+  const int f = (int)mv_from(m), t = (int)mv_to(m);
+  const u32 ty = (pf >> 1) & 7;
+  if (ty == PAWN || ((mv_pcpt(m) >> 1) & 7) != PNONE) return 0;
+  if (ty == KING && (f - t == 2 || t - f == 2)) return 0;
+  return hmc + 1;
+}
+
+}  // namespace zc

@@ -1,0 +1,501 @@
+// perft.hip -- batched perft on gfx950.
+//
+// Replaces perft_gpu (src/zetaperft.cl:801-1477), which walks ONE root with
+// one 64-lane work-group, and its host driver (src/search.c:225-271,
+// src/clrun.c:979-1147).  Design: a level-synchronous frontier over all roots
+// of the batch, one lane per position:
+//   level 0 = roots (SoA in HBM) with their legal-move counts;
+//   count -> exclusive scan -> expand writes (parent, move) pairs of level L+1
+//   -> apply materialises level L+1 (SoA) and counts its moves;
+//   the last level is never materialised: perft_leaf applies each move to its
+//   parent and adds the child's legal-move count (bulk counting at depth-1)
+//   to its root's u64 counter with one wavefront-aggregated atomic per root.
+// Every leaf-level lane does the same branch-free set-wise counting, so the
+// uneven subtree sizes of a batch cost no SIMD divergence.  Levels that would
+// exceed the frontier budget are split into chunks (depth-first over chunks).
+#include <string.h>
+
+#include <vector>
+
+#include "chess.h"
+#include "runtime.h"
+
+using namespace zc;
+
+namespace {
+
+struct Front {  // a frontier level, structure of arrays (coalesced per field)
+  u64 *b0, *b1, *b2, *b3, *cr;
+  u32 *meta;  // root index << 1 | side to move
+  u32 *cnt;   // number of legal moves
+};
+
+__device__ __forceinline__ Pos load_pos(const Front &f, size_t i) {
+  Pos p;
+  p.b0 = f.b0[i];
+  p.b1 = f.b1[i];
+  p.b2 = f.b2[i];
+  p.b3 = f.b3[i];
+  p.cr = f.cr[i];
+  p.stm = f.meta[i] & 1u;
+  return p;
+}
+
+__device__ __forceinline__ void store_pos(const Front &f, size_t i, const Pos &p, u32 root) {
+  f.b0[i] = p.b0;
+  f.b1[i] = p.b1;
+  f.b2[i] = p.b2;
+  f.b3[i] = p.b3;
+  f.cr[i] = p.cr;
+  f.meta[i] = (root << 1) | p.stm;
+}
+
+constexpr int TPB = 256;
+
+__global__ __launch_bounds__(TPB) void k_roots(const u64 *__restrict__ in, int n, Front f) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  const u64 *q = in + (size_t)i * 8;  // zeta_pos = 7 x u64 + stm + pad
+  Pos p;
+  p.b0 = q[0];
+  p.b1 = q[1];
+  p.b2 = q[2];
+  p.b3 = q[3];
+  p.cr = q[4];
+  p.stm = (u32)(q[7] & 1u);
+  store_pos(f, i, p, (u32)i);
+  Legal L;
+  analyse(p, L);
+  f.cnt[i] = (u32)count_legal(p, L, false);
+}
+
+// ---- exclusive scan of u32 counts into u64 offsets (off[n] = total)
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+
+__device__ __forceinline__ u64 block_exscan(u64 v, u64 *lds, u64 &total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  u64 x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    u64 y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[w] = x;
+  __syncthreads();
+  u64 wo = 0;
+  for (int k = 0; k < w; k++) wo += lds[k];
+  total = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return wo + x - v;
+}
+
+__global__ __launch_bounds__(TPB) void k_scan_reduce(const u32 *__restrict__ cnt, size_t n, u64 *bsum) {
+  __shared__ u64 lds[4];
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  u64 s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++)
+    if (base + k < n) s += cnt[base + k];
+  u64 tot;
+  block_exscan(s, lds, tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(TPB) void k_scan_top(u64 *bsum, size_t nb, u64 *total_out) {
+  __shared__ u64 lds[4];
+  u64 carry = 0;
+  for (size_t t0 = 0; t0 < nb; t0 += TPB) {
+    const size_t i = t0 + threadIdx.x;
+    u64 v = i < nb ? bsum[i] : 0;
+    u64 tot;
+    u64 ex = block_exscan(v, lds, tot);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ __launch_bounds__(TPB) void k_scan_down(const u32 *__restrict__ cnt, size_t n, const u64 *bsum,
+                                                   u64 *off) {
+  __shared__ u64 lds[4];
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  u32 v[SCAN_ITEMS];
+  u64 s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    v[k] = base + k < n ? cnt[base + k] : 0u;
+    s += v[k];
+  }
+  u64 tot;
+  u64 ex = block_exscan(s, lds, tot) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < n) off[base + k] = ex;
+    ex += v[k];
+  }
+  // the thread covering element n-1 also writes the total into off[n]
+  if (base < n && base + SCAN_ITEMS >= n) off[n] = ex;
+}
+
+// ---- expand: (parent, move) pairs of every legal move, at the scanned offsets
+__global__ __launch_bounds__(TPB) void k_expand(Front f, size_t i0, size_t i1, const u64 *__restrict__ off,
+                                                u64 base, u32 *__restrict__ par, u32 *__restrict__ mv) {
+  const size_t i = i0 + (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= i1) return;
+  const Pos p = load_pos(f, i);
+  Legal L;
+  analyse(p, L);
+  u64 o = off[i] - base;
+  u64 pcs = L.own;
+  while (pcs) {
+    const int sq = lsb(pcs);
+    pcs &= pcs - 1;
+    u64 t = piece_targets(p, L, sq, false);
+    while (t) {
+      const int to = lsb(t);
+      t &= t - 1;
+      par[o] = (u32)i;
+      mv[o] = make_code(p, sq, to);
+      o++;
+    }
+  }
+}
+
+// ---- apply: materialise the next level and count its moves
+__global__ __launch_bounds__(TPB) void k_apply(Front f, const u32 *__restrict__ par, const u32 *__restrict__ mv,
+                                               size_t m, Front g) {
+  const size_t j = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (j >= m) return;
+  const u32 pi = par[j];
+  const Pos p = load_pos(f, pi);
+  const Pos c = apply(p, mv[j], true);
+  store_pos(g, j, c, f.meta[pi] >> 1);
+  Legal L;
+  analyse(c, L);
+  g.cnt[j] = (u32)count_legal(c, L, false);
+}
+
+__device__ __forceinline__ u64 wave_sum(u64 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---- leaf level: apply + count, accumulate per root
+__global__ __launch_bounds__(TPB) void k_leaf(Front f, const u32 *__restrict__ par, const u32 *__restrict__ mv,
+                                              size_t m, u64 *__restrict__ rootcnt, int nobulk) {
+  const size_t j = (size_t)blockIdx.x * TPB + threadIdx.x;
+  bool active = j < m;
+  u64 v = 0;
+  u32 root = 0;
+  if (active) {
+    const u32 pi = par[j];
+    const Pos p = load_pos(f, pi);
+    const Pos c = apply(p, mv[j], true);
+    Legal L;
+    analyse(c, L);
+    if (nobulk) {
+      // the reference visits every leaf: it generates the leaf's (quiescence)
+      // moves and counts one node (src/zetaperft.cl:1316-1326)
+      const int n = count_legal(c, L, true);
+      v = 1ULL + (u64)(n > 4096);
+    } else {
+      v = (u64)count_legal(c, L, false);
+    }
+    root = f.meta[pi] >> 1;
+  }
+  // wavefront-aggregated accumulation: one atomic per distinct root in the wave
+  const int lane = threadIdx.x & 63;
+  u64 pending = __ballot(active);
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const u32 rl = __shfl(root, leader, 64);
+    const bool mine = active && root == rl;
+    const u64 mask = __ballot(mine);
+    const u64 s = wave_sum(mine ? v : 0ULL);
+    if (lane == leader) atomicAdd((unsigned long long *)&rootcnt[rl], (unsigned long long)s);
+    pending &= ~mask;
+    active = active && !mine;
+  }
+}
+
+__global__ void k_u32_to_u64(const u32 *a, u64 *b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_fill_u64(u64 *b, int n, u64 v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = v;
+}
+
+// ---- playouts (workload generator), one lane per position
+__global__ __launch_bounds__(TPB) void k_playout(u64 seed, int n, int minply, int maxply, u64 *out,
+                                                 u32 *fails) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  Pos s0;  // the initial position, setboard of the standard FEN with KQkq
+  s0.b0 = 0xFFFF000000000000ULL;
+  s0.b1 = 0x91FF00000000FF91ULL;
+  s0.b2 = 0x5A0000000000005AULL;
+  s0.b3 = 0xAD000000000000ADULL;
+  s0.cr = 0x6EFFFFFFFFFFFF6EULL;
+  s0.stm = 0;
+  for (u64 a = 0; a <= 1000; a++) {
+    u64 st = seed ^ ((u64)i * 0x9E3779B97F4A7C15ULL) ^ (a * 0xD1B54A32D192ED03ULL);
+    auto rnd = [&st]() {
+      u64 z = (st += 0x9E3779B97F4A7C15ULL);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      return z ^ (z >> 31);
+    };
+    const int plies = minply + (int)(rnd() % (u64)(maxply - minply + 1));
+    Pos p = s0;
+    u32 hmc = 0;
+    bool ok = true;
+    for (int k = 0; k < plies; k++) {
+      Legal L;
+      analyse(p, L);
+      const int nl = count_legal(p, L, false);
+      if (nl == 0 || hmc >= 100) {
+        ok = false;
+        break;
+      }
+      const int pick = (int)(rnd() % (u64)nl);
+      const u32 m = nth_legal(p, L, pick);
+      hmc = next_hmc(hmc, m);
+      p = apply(p, m, true);
+    }
+    if (ok) {
+      Legal L;
+      analyse(p, L);
+      if (count_legal(p, L, false) == 0 || hmc >= 100) ok = false;
+    }
+    if (ok) {
+      u64 *q = out + (size_t)i * 8;
+      q[0] = p.b0;
+      q[1] = p.b1;
+      q[2] = p.b2;
+      q[3] = p.b3;
+      q[4] = p.cr;
+      q[5] = hash_full(p);
+      q[6] = hmc;
+      q[7] = p.stm;
+      return;
+    }
+  }
+  atomicAdd(fails, 1u);
+}
+
+// ---- host driver
+constexpr int MAXLEV = 16;
+
+struct LevelBufs {
+  zr::DevBuf b0, b1, b2, b3, cr, meta, cnt;  // SoA positions
+  zr::DevBuf off, bsum;                      // scan
+  zr::DevBuf par, mv;                        // pairs of the next level
+  size_t cap = 0, paircap = 0;
+};
+
+struct PerftState {
+  LevelBufs lev[MAXLEV];
+  zr::DevBuf total, roots, counts, fails;
+};
+PerftState g_ps;
+
+int ensure_level(int L, size_t n) {
+  LevelBufs &b = g_ps.lev[L];
+  if (b.cap >= n && b.cap) return ZETA_OK;
+  size_t cap = n + n / 4 + 1024;
+  int r;
+  if ((r = zr::ensure(b.b0, cap * 8)) || (r = zr::ensure(b.b1, cap * 8)) || (r = zr::ensure(b.b2, cap * 8)) ||
+      (r = zr::ensure(b.b3, cap * 8)) || (r = zr::ensure(b.cr, cap * 8)) || (r = zr::ensure(b.meta, cap * 4)) ||
+      (r = zr::ensure(b.cnt, cap * 4)) || (r = zr::ensure(b.off, (cap + 1) * 8)) ||
+      (r = zr::ensure(b.bsum, (cap / SCAN_TILE + 2) * 8)))
+    return r;
+  b.cap = cap;
+  return ZETA_OK;
+}
+
+int ensure_pairs(int L, size_t m) {
+  LevelBufs &b = g_ps.lev[L];
+  if (b.paircap >= m && b.paircap) return ZETA_OK;
+  size_t cap = m + m / 8 + 1024;
+  int r;
+  if ((r = zr::ensure(b.par, cap * 4)) || (r = zr::ensure(b.mv, cap * 4))) return r;
+  b.paircap = cap;
+  return ZETA_OK;
+}
+
+Front front_of(int L) {
+  LevelBufs &b = g_ps.lev[L];
+  return Front{(u64 *)b.b0.p, (u64 *)b.b1.p, (u64 *)b.b2.p, (u64 *)b.b3.p,
+               (u64 *)b.cr.p, (u32 *)b.meta.p, (u32 *)b.cnt.p};
+}
+
+inline unsigned blocks(size_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+// exclusive scan of level L's counts [i0, i1) into lev[L].off[0..n]; returns total
+int scan_counts(int L, size_t i0, size_t i1, u64 *total) {
+  LevelBufs &b = g_ps.lev[L];
+  const size_t n = i1 - i0;
+  const size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  const u32 *cnt = (const u32 *)b.cnt.p + i0;
+  if (n == 0) {
+    *total = 0;
+    return ZETA_OK;
+  }
+  ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_reduce, (unsigned)nb, TPB, 0, cnt, n, (u64 *)b.bsum.p);
+  ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_top, 1, TPB, 0, (u64 *)b.bsum.p, nb, (u64 *)g_ps.total.p);
+  ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_down, (unsigned)nb, TPB, 0, cnt, n, (const u64 *)b.bsum.p, (u64 *)b.off.p);
+  ZCHECK(hipMemcpyAsync(total, g_ps.total.p, 8, hipMemcpyDeviceToHost, zr::ctx().stream));
+  ZCHECK(hipStreamSynchronize(zr::ctx().stream));
+  return ZETA_OK;
+}
+
+int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
+  u64 total = 0;
+  int r = scan_counts(L, i0, i1, &total);
+  if (r) return r;
+  if (total == 0) return ZETA_OK;
+  const u64 budget = zr::ctx().cfg.frontier_budget;
+  // chunk boundaries (relative to i0) so that each chunk's pairs fit the budget
+  std::vector<size_t> cuts;
+  cuts.push_back(0);
+  if (total > budget) {
+    std::vector<u64> off(i1 - i0 + 1);
+    ZCHECK(hipMemcpyAsync(off.data(), g_ps.lev[L].off.p, off.size() * 8, hipMemcpyDeviceToHost,
+                          zr::ctx().stream));
+    ZCHECK(hipStreamSynchronize(zr::ctx().stream));
+    size_t a = 0;
+    for (size_t k = 1; k <= i1 - i0; k++) {
+      if (off[k] - off[a] > budget) {
+        if (k - 1 == a) return zr::fail(ZETA_ENOMEM, "frontier budget smaller than one node's children");
+        cuts.push_back(k - 1);
+        a = k - 1;
+      }
+    }
+    cuts.push_back(i1 - i0);
+    // chunks re-read their own offsets, so keep the level's offset array intact
+    for (size_t c = 0; c + 1 < cuts.size(); c++) {
+      const size_t a0 = cuts[c], a1 = cuts[c + 1];
+      const u64 m = off[a1] - off[a0];
+      if ((r = ensure_pairs(L, m))) return r;
+      ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, TPB), TPB, 0, front_of(L), i0 + a0, i0 + a1,
+              (const u64 *)g_ps.lev[L].off.p - i0, (u64)off[a0], (u32 *)g_ps.lev[L].par.p,
+              (u32 *)g_ps.lev[L].mv.p);
+      if (L + 1 == D - 1) {
+        ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u32 *)g_ps.lev[L].par.p,
+                (const u32 *)g_ps.lev[L].mv.p, (size_t)m, d_counts, nobulk);
+      } else {
+        if ((r = ensure_level(L + 1, m))) return r;
+        ZLAUNCH(zr::K_PERFT_APPLY, m, k_apply, blocks(m, TPB), TPB, 0, front_of(L),
+                (const u32 *)g_ps.lev[L].par.p, (const u32 *)g_ps.lev[L].mv.p, (size_t)m, front_of(L + 1));
+        if ((r = process(L + 1, 0, m, D, d_counts, nobulk))) return r;
+      }
+    }
+    return ZETA_OK;
+  }
+  if ((r = ensure_pairs(L, total))) return r;
+  ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, TPB), TPB, 0, front_of(L), i0, i1,
+          (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u32 *)g_ps.lev[L].par.p, (u32 *)g_ps.lev[L].mv.p);
+  if (L + 1 == D - 1) {
+    ZLAUNCH(zr::K_PERFT_LEAF, total, k_leaf, blocks(total, TPB), TPB, 0, front_of(L),
+            (const u32 *)g_ps.lev[L].par.p, (const u32 *)g_ps.lev[L].mv.p, (size_t)total, d_counts, nobulk);
+    return ZETA_OK;
+  }
+  if ((r = ensure_level(L + 1, total))) return r;
+  ZLAUNCH(zr::K_PERFT_APPLY, total, k_apply, blocks(total, TPB), TPB, 0, front_of(L),
+          (const u32 *)g_ps.lev[L].par.p, (const u32 *)g_ps.lev[L].mv.p, (size_t)total, front_of(L + 1));
+  return process(L + 1, 0, total, D, d_counts, nobulk);
+}
+
+int perft_device(const void *d_pos, int n, int depth, u64 *d_counts, int flags) {
+  const int nobulk = (flags & ZETA_PERFT_NOBULK) ? 1 : 0;
+  if (n < 0 || depth < 0) return zr::fail(ZETA_EINVAL, "perft: n=%d depth=%d", n, depth);
+  if (n == 0) return ZETA_OK;
+  const int D = nobulk ? depth + 1 : depth;
+  if (D >= MAXLEV) return zr::fail(ZETA_EINVAL, "perft depth %d exceeds %d", depth, MAXLEV - 2);
+  int r;
+  if ((r = zr::ensure(g_ps.total, 64))) return r;
+  if (depth == 0) {
+    ZLAUNCH(zr::K_PERFT_ROOTS, n, k_fill_u64, blocks(n, TPB), TPB, 0, d_counts, n, (u64)1);
+    return ZETA_OK;
+  }
+  if ((r = ensure_level(0, (size_t)n))) return r;
+  ZLAUNCH(zr::K_PERFT_ROOTS, n, k_roots, blocks(n, TPB), TPB, 0, (const u64 *)d_pos, n, front_of(0));
+  if (D == 1) {
+    ZLAUNCH(zr::K_PERFT_ROOTS, n, k_u32_to_u64, blocks(n, TPB), TPB, 0, (const u32 *)g_ps.lev[0].cnt.p, d_counts,
+            n);
+    return ZETA_OK;
+  }
+  ZLAUNCH(zr::K_PERFT_ROOTS, n, k_fill_u64, blocks(n, TPB), TPB, 0, d_counts, n, (u64)0);
+  return process(0, 0, (size_t)n, D, d_counts, nobulk);
+}
+
+}  // namespace
+
+void perft_release() {
+  for (auto &l : g_ps.lev) {
+    zr::release(l.b0); zr::release(l.b1); zr::release(l.b2); zr::release(l.b3);
+    zr::release(l.cr); zr::release(l.meta); zr::release(l.cnt); zr::release(l.off);
+    zr::release(l.bsum); zr::release(l.par); zr::release(l.mv);
+    l.cap = l.paircap = 0;
+  }
+  zr::release(g_ps.total);
+  zr::release(g_ps.roots);
+  zr::release(g_ps.counts);
+  zr::release(g_ps.fails);
+}
+
+extern "C" int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts, int flags) {
+  int r = zr::require_up();
+  if (r) return r;
+  if ((!d_pos || !d_counts) && n > 0) return zr::fail(ZETA_EINVAL, "perft: null pointer");
+  r = perft_device(d_pos, n, depth, (u64 *)d_counts, flags);
+  hipError_t e = hipStreamSynchronize(zr::ctx().stream);
+  zr::time_collect();
+  if (r) return r;
+  if (e != hipSuccess) return zr::hip_fail(e, "perft sync", __FILE__, __LINE__);
+  return ZETA_OK;
+}
+
+extern "C" int zeta_perft_batch(const zeta_pos *pos, int n, int depth, uint64_t *counts) {
+  int r = zr::require_up();
+  if (r) return r;
+  if (n < 0 || (n > 0 && (!pos || !counts))) return zr::fail(ZETA_EINVAL, "perft: bad arguments");
+  if (n == 0) return ZETA_OK;
+  if ((r = zr::ensure(g_ps.roots, (size_t)n * sizeof(zeta_pos))) || (r = zr::ensure(g_ps.counts, (size_t)n * 8)))
+    return r;
+  ZCHECK(hipMemcpyAsync(g_ps.roots.p, pos, (size_t)n * sizeof(zeta_pos), hipMemcpyHostToDevice, zr::ctx().stream));
+  r = perft_device(g_ps.roots.p, n, depth, (u64 *)g_ps.counts.p, 0);
+  if (r) {
+    (void)hipStreamSynchronize(zr::ctx().stream);
+    zr::time_collect();
+    return r;
+  }
+  ZCHECK(hipMemcpyAsync(counts, g_ps.counts.p, (size_t)n * 8, hipMemcpyDeviceToHost, zr::ctx().stream));
+  ZCHECK(hipStreamSynchronize(zr::ctx().stream));
+  zr::time_collect();
+  return ZETA_OK;
+}
+
+extern "C" int zeta_playouts(uint64_t seed, int n, int minply, int maxply, zeta_pos *out) {
+  int r = zr::require_up();
+  if (r) return r;
+  if (n < 0 || minply < 0 || maxply < minply || (n > 0 && !out))
+    return zr::fail(ZETA_EINVAL, "playouts: bad arguments");
+  if (n == 0) return ZETA_OK;
+  if ((r = zr::ensure(g_ps.roots, (size_t)n * sizeof(zeta_pos))) || (r = zr::ensure(g_ps.fails, 4))) return r;
+  ZCHECK(hipMemsetAsync(g_ps.fails.p, 0, 4, zr::ctx().stream));
+  ZLAUNCH(zr::K_PLAYOUT, n, k_playout, blocks(n, TPB), TPB, 0, seed, n, minply, maxply, (u64 *)g_ps.roots.p,
+          (u32 *)g_ps.fails.p);
+  u32 fails = 0;
+  ZCHECK(hipMemcpyAsync(out, g_ps.roots.p, (size_t)n * sizeof(zeta_pos), hipMemcpyDeviceToHost, zr::ctx().stream));
+  ZCHECK(hipMemcpyAsync(&fails, g_ps.fails.p, 4, hipMemcpyDeviceToHost, zr::ctx().stream));
+  ZCHECK(hipStreamSynchronize(zr::ctx().stream));
+  zr::time_collect();
+  if (fails) return zr::fail(ZETA_EINVAL, "playouts: %u positions found no acceptable playout", fails);
+  return ZETA_OK;
+}

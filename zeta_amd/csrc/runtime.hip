@@ -1,0 +1,207 @@
+// runtime.hip -- context, errors, device buffers and kernel timing.
+// Replaces the reference's OpenCL set-up (src/clrun.c:51-514 cl_init_device,
+// src/clrun.c:1268-1490 cl_release_device): one device, one in-order stream.
+#include <stdarg.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "runtime.h"
+
+namespace zr {
+
+static Ctx g_ctx;
+static char g_err[1024] = "";
+
+Ctx &ctx() { return g_ctx; }
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  if (g_ctx.cfg.verbose) fprintf(stderr, "zeta: %s\n", g_err);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  return fail(ZETA_EHIP, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+}
+
+int require_up() {
+  if (!g_ctx.up) return fail(ZETA_ESTATE, "library not initialised (call zeta_init)");
+  return ZETA_OK;
+}
+
+int ensure(DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return ZETA_OK;
+  if (b.p) {
+    // work queued on the library stream may still read the old buffer
+    if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
+    hipError_t e = hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (e != hipSuccess) return hip_fail(e, "hipFree", __FILE__, __LINE__);
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(ZETA_ENOMEM, "hipMalloc of %zu bytes failed (%s)", want, hipGetErrorString(e));
+  }
+  b.bytes = want;
+  return ZETA_OK;
+}
+
+void release(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+static const char *kname(int id) {
+  static const char *names[K_COUNT_IDS] = {"perft_roots", "perft_count", "perft_scan", "perft_expand",
+                                           "perft_apply", "perft_leaf",  "playout",    "search",
+                                           "mlp_fwd",     "mlp_bwd",     "mlp_adam",   "mlp_score"};
+  return names[id];
+}
+
+static hipEvent_t get_event() {
+  if (!g_ctx.event_pool.empty()) {
+    hipEvent_t e = g_ctx.event_pool.back();
+    g_ctx.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void time_begin(int id, uint64_t items, hipEvent_t *a, hipEvent_t *b) {
+  *a = *b = nullptr;
+  if (!g_ctx.timing) return;
+  *a = get_event();
+  *b = get_event();
+  if (*a) (void)hipEventRecord(*a, g_ctx.stream);
+  (void)id;
+  (void)items;
+}
+
+void time_end(hipEvent_t a, hipEvent_t b, int id, uint64_t items) {
+  if (!g_ctx.timing || !a || !b) return;
+  (void)hipEventRecord(b, g_ctx.stream);
+  g_ctx.pending.push_back({id, a, b, items});
+}
+
+void time_collect() {
+  for (auto &p : g_ctx.pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      zeta_kstat &k = g_ctx.kstat[p.id];
+      k.launches += 1;
+      k.ms += ms;
+      k.items += p.items;
+    }
+    g_ctx.event_pool.push_back(p.a);
+    g_ctx.event_pool.push_back(p.b);
+  }
+  g_ctx.pending.clear();
+}
+
+}  // namespace zr
+
+using namespace zr;
+
+extern "C" int zeta_init(const zeta_config *cfg) {
+  Ctx &c = ctx();
+  if (c.up) return ZETA_OK;
+  zeta_config d{};
+  if (cfg) d = *cfg;
+  if (d.frontier_budget == 0) d.frontier_budget = 1ULL << 28;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return fail(ZETA_ENODEV, "no HIP device available (%s)", e == hipSuccess ? "count 0" : hipGetErrorString(e));
+  }
+  if (d.device < 0 || d.device >= n) return fail(ZETA_EINVAL, "device %d out of range (%d devices)", d.device, n);
+  e = hipSetDevice(d.device);
+  if (e != hipSuccess) return fail(ZETA_ENODEV, "hipSetDevice(%d): %s", d.device, hipGetErrorString(e));
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, d.device);
+  if (e != hipSuccess) return fail(ZETA_ENODEV, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+  c.arch = prop.gcnArchName;
+  if (c.arch.compare(0, 6, "gfx950") != 0)
+    return fail(ZETA_ENODEV, "device %d is %s; this library is built for gfx950 only", d.device, prop.gcnArchName);
+  c.cu_count = prop.multiProcessorCount;
+  e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(ZETA_ENODEV, "hipStreamCreate: %s", hipGetErrorString(e));
+  c.cfg = d;
+  c.device = d.device;
+  memset(c.kstat, 0, sizeof(c.kstat));
+  c.up = true;
+  return ZETA_OK;
+}
+
+// release hooks of the subsystems (defined in their own files)
+void perft_release();
+
+extern "C" int zeta_shutdown(void) {
+  Ctx &c = ctx();
+  if (!c.up) return ZETA_OK;
+  (void)hipStreamSynchronize(c.stream);
+  time_collect();
+  perft_release();
+  for (auto e : c.event_pool) (void)hipEventDestroy(e);
+  c.event_pool.clear();
+  (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+  c.up = false;
+  return ZETA_OK;
+}
+
+extern "C" int zeta_last_error(char *buf, size_t n) {
+  if (!buf || n == 0) return ZETA_EINVAL;
+  snprintf(buf, n, "%s", zr::g_err);
+  return ZETA_OK;
+}
+
+extern "C" int zeta_device_info(char *buf, size_t n) {
+  int r = require_up();
+  if (r) return r;
+  if (!buf || n == 0) return ZETA_EINVAL;
+  snprintf(buf, n, "%s cu=%d", ctx().arch.c_str(), ctx().cu_count);
+  return ZETA_OK;
+}
+
+extern "C" int zeta_set_timing(int on) {
+  int r = require_up();
+  if (r) return r;
+  ctx().timing = on != 0;
+  return ZETA_OK;
+}
+
+extern "C" int zeta_stats_get(zeta_stats *st) {
+  int r = require_up();
+  if (r) return r;
+  if (!st) return ZETA_EINVAL;
+  (void)hipStreamSynchronize(ctx().stream);
+  time_collect();
+  memset(st, 0, sizeof(*st));
+  for (int i = 0; i < K_COUNT_IDS && i < ZETA_MAX_KSTATS; i++) {
+    st->k[i] = ctx().kstat[i];
+    snprintf(st->k[i].name, sizeof(st->k[i].name), "%s", kname(i));
+  }
+  st->nk = K_COUNT_IDS < ZETA_MAX_KSTATS ? K_COUNT_IDS : ZETA_MAX_KSTATS;
+  return ZETA_OK;
+}
+
+extern "C" int zeta_stats_reset(void) {
+  int r = require_up();
+  if (r) return r;
+  (void)hipStreamSynchronize(ctx().stream);
+  time_collect();
+  memset(ctx().kstat, 0, sizeof(ctx().kstat));
+  return ZETA_OK;
+}
