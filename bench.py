@@ -61,37 +61,36 @@ def main():
               "host-side sum; no multi-GPU run is part of it)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
+    # torch is not imported: its wheel carries its own HIP runtime, and two HIP
+    # runtimes in one process do not share the device.  Device memory and
+    # synchronisation go through the library's C-ABI instead.
     import zeta_amd
 
     zeta_amd.init()
     pos = zeta_amd.playouts(2, args.positions, 8, 40)
-    d_pos = torch.from_numpy(pos.view(np.uint8).copy()).cuda()
-    d_cnt = torch.zeros(args.positions, dtype=torch.int64, device="cuda")
-    torch.cuda.synchronize()
+    d_pos = zeta_amd.DeviceBuffer(pos.nbytes)
+    d_cnt = zeta_amd.DeviceBuffer(8 * args.positions)
+    d_pos.upload(pos)
 
     def step():
-        zeta_amd.perft_device(d_pos.data_ptr(), args.positions, args.depth, d_cnt.data_ptr(), nobulk=args.nobulk)
+        zeta_amd.perft_device(d_pos.ptr, args.positions, args.depth, d_cnt.ptr, nobulk=args.nobulk)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    counts = d_cnt.cpu().numpy().view(np.uint64).copy()
+    zeta_amd.synchronize()
+    counts = d_cnt.download(np.zeros(args.positions, dtype=np.uint64))
     leaves = int(counts.sum())
     zeta_amd.set_timing(True)
-    for _ in range(1):
-        step()
-    torch.cuda.synchronize()
+    step()
     kst = zeta_amd.stats()
     zeta_amd.set_timing(False)
     zeta_amd.stats_reset()
 
-    torch.cuda.synchronize()
+    zeta_amd.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    torch.cuda.synchronize()
+    zeta_amd.synchronize()
     dt = time.perf_counter() - t0
     ms = dt * 1e3 / args.steps
     value = leaves * args.steps / dt

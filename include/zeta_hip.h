@@ -81,6 +81,16 @@ int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts,
  * side has no legal move or the half-move clock reaches 100. */
 int zeta_playouts(uint64_t seed, int n, int minply, int maxply, zeta_pos *out);
 
+/* ------------------------------------------------------------ device memory
+ * Device buffers for callers that keep inputs resident on the GPU (the
+ * *_device entry points).  Plain pointers and sizes; all copies are
+ * synchronous with respect to the library's stream. */
+int zeta_dev_alloc(void **dptr, size_t bytes);
+int zeta_dev_free(void *dptr);
+int zeta_memcpy_htod(void *dst, const void *src, size_t bytes);
+int zeta_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int zeta_synchronize(void);
+
 /* ------------------------------------------------------------------ stats */
 #define ZETA_MAX_KSTATS 16
 typedef struct {

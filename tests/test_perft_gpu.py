@@ -56,13 +56,15 @@ def test_config2_perft4_golden(zeta, oracle):
 
 
 def test_nobulk_equals_bulk(zeta, oracle):
-    import torch
     pos = oracle.gen_playouts(2, 256, 8, 40)
-    d_pos = torch.from_numpy(pos.view(np.uint8).copy()).cuda()
-    d_cnt = torch.zeros(len(pos), dtype=torch.int64, device="cuda")
-    zeta.perft_device(d_pos.data_ptr(), len(pos), 3, d_cnt.data_ptr(), nobulk=True)
-    torch.cuda.synchronize()
-    assert (d_cnt.cpu().numpy().view(np.uint64) == oracle.perft_batch(pos, 3)).all()
+    d_pos = zeta.DeviceBuffer(pos.nbytes)
+    d_cnt = zeta.DeviceBuffer(8 * len(pos))
+    d_pos.upload(pos)
+    zeta.perft_device(d_pos.ptr, len(pos), 3, d_cnt.ptr, nobulk=True)
+    got = d_cnt.download(np.zeros(len(pos), dtype=np.uint64))
+    d_pos.free()
+    d_cnt.free()
+    assert (got == oracle.perft_batch(pos, 3)).all()
 
 
 def test_edge_cases(zeta, oracle):

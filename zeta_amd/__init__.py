@@ -63,9 +63,14 @@ def lib():
         L.zeta_playouts.argtypes = [u64, i32, i32, i32, vp]
         L.zeta_stats_get.argtypes = [ctypes.POINTER(_Stats)]
         L.zeta_set_timing.argtypes = [i32]
+        L.zeta_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        L.zeta_dev_free.argtypes = [vp]
+        L.zeta_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
+        L.zeta_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
         for f in ("zeta_init", "zeta_shutdown", "zeta_last_error", "zeta_device_info",
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
-                  "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing"):
+                  "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
+                  "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -141,6 +146,37 @@ def playouts(seed, n, minply, maxply):
     out = np.zeros(n, dtype=POS_DTYPE)
     _check(lib().zeta_playouts(int(seed), int(n), int(minply), int(maxply), _ptr(out)), "zeta_playouts")
     return out
+
+
+class DeviceBuffer:
+    """Device memory owned through the C-ABI (zeta_dev_alloc / zeta_dev_free)."""
+
+    def __init__(self, nbytes):
+        ensure_init()
+        p = ctypes.c_void_p()
+        _check(lib().zeta_dev_alloc(ctypes.byref(p), int(nbytes)), "zeta_dev_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().zeta_memcpy_htod(ctypes.c_void_p(self.ptr), _ptr(arr), arr.nbytes), "zeta_memcpy_htod")
+
+    def download(self, arr):
+        assert arr.flags["C_CONTIGUOUS"] and arr.nbytes <= self.nbytes
+        _check(lib().zeta_memcpy_dtoh(_ptr(arr), ctypes.c_void_p(self.ptr), arr.nbytes), "zeta_memcpy_dtoh")
+        return arr
+
+    def free(self):
+        if self.ptr:
+            _check(lib().zeta_dev_free(ctypes.c_void_p(self.ptr)), "zeta_dev_free")
+            self.ptr = None
+
+
+def synchronize():
+    ensure_init()
+    _check(lib().zeta_synchronize(), "zeta_synchronize")
 
 
 def set_timing(on=True):

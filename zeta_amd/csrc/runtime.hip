@@ -205,3 +205,52 @@ extern "C" int zeta_stats_reset(void) {
   memset(ctx().kstat, 0, sizeof(ctx().kstat));
   return ZETA_OK;
 }
+
+extern "C" int zeta_dev_alloc(void **dptr, size_t bytes) {
+  int r = require_up();
+  if (r) return r;
+  if (!dptr) return fail(ZETA_EINVAL, "zeta_dev_alloc: null out pointer");
+  *dptr = nullptr;
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    return fail(ZETA_ENOMEM, "hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+  }
+  return ZETA_OK;
+}
+
+extern "C" int zeta_dev_free(void *dptr) {
+  int r = require_up();
+  if (r) return r;
+  if (!dptr) return ZETA_OK;
+  ZCHECK(hipStreamSynchronize(ctx().stream));
+  ZCHECK(hipFree(dptr));
+  return ZETA_OK;
+}
+
+extern "C" int zeta_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  int r = require_up();
+  if (r) return r;
+  if (bytes && (!dst || !src)) return fail(ZETA_EINVAL, "zeta_memcpy_htod: null pointer");
+  ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx().stream));
+  ZCHECK(hipStreamSynchronize(ctx().stream));
+  return ZETA_OK;
+}
+
+extern "C" int zeta_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  int r = require_up();
+  if (r) return r;
+  if (bytes && (!dst || !src)) return fail(ZETA_EINVAL, "zeta_memcpy_dtoh: null pointer");
+  ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
+  ZCHECK(hipStreamSynchronize(ctx().stream));
+  return ZETA_OK;
+}
+
+extern "C" int zeta_synchronize(void) {
+  int r = require_up();
+  if (r) return r;
+  ZCHECK(hipStreamSynchronize(ctx().stream));
+  time_collect();
+  return ZETA_OK;
+}
