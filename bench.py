@@ -24,15 +24,12 @@ VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
 HBM_PEAK = 8.0e12
 
 
-def ops_per_position():
-    """Algorithmic 64-bit integer ops of one leaf-kernel position (apply +
-    analyse + count), counted once by the op-counting host build of the
-    device core (tests/hostcore/opcount.cpp) over this workload's leaf level."""
-    path = os.path.join(ROOT, "zeta_amd", "ops_per_position.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f)
-    return None
+# Algorithmic work of one leaf-kernel position (apply + analyse + count), in
+# 64-bit integer operations, counted by the op-counting host build of the
+# device core (tests/hostcore/opcount.cpp, `opcount 3` over 64 workload roots,
+# 2,115,331 leaf-level positions): bulk = full legal-move count of the node,
+# nobulk = the quiescence move count the reference does at each visited leaf.
+OPS_PER_POSITION = {"bulk": 931.5, "nobulk": 890.6}
 
 
 def cpu_baseline(pos, depth, threads):
@@ -98,8 +95,8 @@ def main():
     # roofline of the dominant kernel, from its HIP-event time (separate timed pass)
     leaf = kst.get("perft_leaf", (0, 0.0, 0))
     roof = None
-    ops = ops_per_position()
-    if leaf[0] and ops:
+    ops = OPS_PER_POSITION
+    if leaf[0]:
         per_launch_items = leaf[2] / leaf[0]
         avg_s = leaf[1] / leaf[0] / 1e3
         key = "nobulk" if args.nobulk else "bulk"
@@ -107,7 +104,7 @@ def main():
         achieved = lane_ops / avg_s / 1e12
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK / 1e12, 2),
                 "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved * 1e12 / VALU_PEAK, 4),
-                "traffic": ops.get("traffic_bytes_per_launch"),
+                "traffic": None,
                 "kernel": "perft_leaf", "avg_launch_ms": round(leaf[1] / leaf[0], 4),
                 "items_per_launch": int(per_launch_items), "ops64_per_item": ops[key]}
 

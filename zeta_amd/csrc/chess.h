@@ -32,7 +32,13 @@
 
 namespace zc {
 
+#ifdef ZC_U64
+typedef ZC_U64 u64;  // op-counting host build (tests/hostcore/opcount.cpp)
+#define ZC_RAW(x) zc_raw(x)
+#else
 typedef uint64_t u64;
+#define ZC_RAW(x) (x)
+#endif
 typedef uint32_t u32;
 typedef int32_t s32;
 
@@ -52,11 +58,11 @@ constexpr u64 SMCRBLACKK = 0x9000000000000000ULL;
 
 enum { PNONE = 0, PAWN = 1, KNIGHT = 2, KING = 3, BISHOP = 4, ROOK = 5, QUEEN = 6 };
 
-ZD int popc(u64 x) { return __builtin_popcountll(x); }
-ZD int lsb(u64 x) { return __builtin_ctzll(x); }
-ZD int msb(u64 x) { return 63 - __builtin_clzll(x); }
+ZD int popc(u64 x) { return __builtin_popcountll(ZC_RAW(x)); }
+ZD int lsb(u64 x) { return __builtin_ctzll(ZC_RAW(x)); }
+ZD int msb(u64 x) { return 63 - __builtin_clzll(ZC_RAW(x)); }
 // Q4: shift counts are taken modulo 64, as OpenCL C does.
-ZD u64 bit(int sq) { return 1ULL << (sq & 63); }
+ZD u64 bit(int sq) { return u64(1) << (sq & 63); }
 ZD u64 shlm(u64 x, int n) { return x << (n & 63); }
 ZD u64 shrm(u64 x, int n) { return x >> (n & 63); }
 ZD u64 sel(bool c, u64 x) { return x & (0ULL - (u64)c); }
