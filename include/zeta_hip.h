@@ -81,6 +81,36 @@ int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts,
  * side has no legal move or the half-move clock reaches 100. */
 int zeta_playouts(uint64_t seed, int n, int minply, int maxply, zeta_pos *out);
 
+/* ----------------------------------------------------------------- search
+ * Fixed-depth alpha-beta + quiescence of n root positions (iterative
+ * deepening 1..depth per root, as rootsearch, src/search.c:77-198).
+ * Replaces cl_run_alphabeta (src/clrun.h:28; src/clrun.c:675-977).
+ * Node semantics: alphabeta_gpu (src/zeta.cl:1319-1939) with null move, LMR
+ * and IID off.  score = PV[0] as rootsearch reports it (src/search.c:137):
+ * the exact minimax value of the depth-limited tree, independent of move
+ * order, TT and worker count.  nodes = moves made (src/zeta.cl:2283-2285). */
+typedef struct {
+  int depth;          /* target depth, 1..40 */
+  int workers;        /* wavefront workers per root (ABDADA helpers), >= 1 */
+  int tt_log2;        /* TT size 2^tt_log2 x 16 B; -1 = zeta_config.tt_log2; 0 = none */
+  int clear_tt;       /* 1 = clear the TT before the batch */
+  uint64_t max_nodes; /* per-worker node budget per root, 0 = none */
+  int use_mlp;        /* learned move ordering at interior nodes (needs zeta_mlp_load) */
+  int mlp_min_depth;  /* nodes with depth >= this use the learned ordering */
+  int log_records;    /* record cutoff examples for training (zeta_records_*) */
+  int reserved;
+} zeta_search_opts;
+
+typedef struct {
+  int32_t score;     /* centipawns, side to move; mate = +-(32000 - plies) */
+  uint32_t bestmove; /* reference move code (src/types.h:67-74) */
+  uint64_t nodes;    /* moves made by all workers on this root */
+  int32_t depth;     /* deepest completed iteration */
+  int32_t status;    /* 0 ok, else a ZETA_E* code */
+} zeta_result;
+
+int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, zeta_result *res);
+
 /* ------------------------------------------------------------ device memory
  * Device buffers for callers that keep inputs resident on the GPU (the
  * *_device entry points).  Plain pointers and sizes; all copies are

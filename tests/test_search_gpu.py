@@ -1,0 +1,63 @@
+"""GPU parity tests of the batched alpha-beta search against the oracle: the
+root score must equal the oracle's exact fixed-depth value for every
+position, whatever the move order, TT use or number of workers."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+
+
+def test_terminal_and_draw_positions(zeta, oracle):
+    fens = ["6k1/5ppp/8/8/8/8/5PPP/R5K1 w - - 0 1",      # mate in one
+            "R5k1/5ppp/8/8/8/8/5PPP/6K1 b - - 0 1",      # mated root
+            "7k/5Q2/6K1/8/8/8/8/8 b - - 0 1",            # stalemated root
+            "8/8/8/8/3k4/8/3q4/3K4 w - - 0 1",           # forced capture to bare kings
+            "7k/8/8/8/8/8/8/R6K w - - 99 80",            # fifty-move rule
+            "7k/8/8/8/8/8/8/R6K w - - 0 80"]
+    pos = np.concatenate([oracle.setboard(f) for f in fens])
+    for d in (1, 2, 3, 4):
+        want = [oracle.search(pos[i:i + 1], d)[0] for i in range(len(pos))]
+        for tt in (0, 20):
+            got = zeta.search(pos, d, tt_log2=tt)
+            print("depth", d, "tt", tt, list(got["score"]), want, flush=True)
+            assert list(got["score"]) == want
+            assert (got["status"] == 0).all()
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3, 4])
+def test_playout_positions_small_depth(zeta, oracle, depth):
+    pos = oracle.gen_playouts(3, 128, 20, 60)
+    want, _ = oracle.search_batch(pos, depth)
+    got = zeta.search(pos, depth, tt_log2=0)
+    bad = np.nonzero(got["score"] != want)[0]
+    print("depth", depth, "mismatches", len(bad), bad[:10], flush=True)
+    assert len(bad) == 0
+    got = zeta.search(pos, depth, tt_log2=22)
+    assert (got["score"] == want).all()
+
+
+def test_workers_and_tt_do_not_change_scores(zeta, oracle):
+    pos = oracle.gen_playouts(3, 64, 20, 60)
+    want, _ = oracle.search_batch(pos, 4)
+    for workers in (1, 4):
+        for tt in (0, 24):
+            got = zeta.search(pos, 4, workers=workers, tt_log2=tt)
+            assert (got["score"] == want).all(), (workers, tt)
+
+
+def test_config3_depth6_golden(zeta, oracle):
+    path = os.path.join(HERE, "golden", "c3_search6.npz")
+    if not os.path.exists(path):
+        pytest.skip("golden depth-6 scores not generated")
+    gold = np.load(path)
+    pos = oracle.gen_playouts(3, len(gold["score6"]), 20, 60)
+    assert (pos["b"][:, 5] == gold["hash"]).all()
+    got = zeta.search(pos, 4, workers=2, tt_log2=26)
+    assert (got["score"] == gold["score4"]).all()
+    got = zeta.search(pos, 6, workers=2, tt_log2=26)
+    bad = np.nonzero(got["score"] != gold["score6"])[0]
+    print("depth 6 mismatches", len(bad), "nodes", int(got["nodes"].sum()), flush=True)
+    assert len(bad) == 0

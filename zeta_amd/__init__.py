@@ -41,6 +41,16 @@ class _Stats(ctypes.Structure):
     _fields_ = [("nk", ctypes.c_int), ("k", _KStat * 16)]
 
 
+class SearchOpts(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_int), ("workers", ctypes.c_int), ("tt_log2", ctypes.c_int),
+                ("clear_tt", ctypes.c_int), ("max_nodes", ctypes.c_uint64), ("use_mlp", ctypes.c_int),
+                ("mlp_min_depth", ctypes.c_int), ("log_records", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+RESULT_DTYPE = np.dtype([("score", "<i4"), ("bestmove", "<u4"), ("nodes", "<u8"), ("depth", "<i4"),
+                         ("status", "<i4")])
+
+
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("tt_log2", ctypes.c_int),
                 ("frontier_budget", ctypes.c_uint64), ("workers", ctypes.c_int),
@@ -63,6 +73,7 @@ def lib():
         L.zeta_playouts.argtypes = [u64, i32, i32, i32, vp]
         L.zeta_stats_get.argtypes = [ctypes.POINTER(_Stats)]
         L.zeta_set_timing.argtypes = [i32]
+        L.zeta_search_batch.argtypes = [vp, i32, ctypes.POINTER(SearchOpts), vp]
         L.zeta_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         L.zeta_dev_free.argtypes = [vp]
         L.zeta_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
@@ -70,7 +81,8 @@ def lib():
         for f in ("zeta_init", "zeta_shutdown", "zeta_last_error", "zeta_device_info",
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
                   "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
-                  "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize"):
+                  "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize",
+                  "zeta_search_batch"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -138,6 +150,20 @@ def perft_device(d_pos, n, depth, d_counts, nobulk=False):
     ensure_init()
     _check(lib().zeta_perft_batch_device(ctypes.c_void_p(d_pos), int(n), int(depth), ctypes.c_void_p(d_counts),
                                          ZETA_PERFT_NOBULK if nobulk else 0), "zeta_perft_batch_device")
+
+
+def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_mlp=False,
+           mlp_min_depth=3, log_records=False):
+    """Fixed-depth alpha-beta + quiescence of every position (iterative
+    deepening 1..depth).  Returns a structured array (score, bestmove, nodes,
+    depth, status) -- the mirror of rootsearch's PV[0]/PV[1]/node count."""
+    ensure_init()
+    pos = as_positions(pos)
+    out = np.zeros(len(pos), dtype=RESULT_DTYPE)
+    o = SearchOpts(int(depth), int(workers), int(tt_log2), int(bool(clear_tt)), int(max_nodes),
+                   int(bool(use_mlp)), int(mlp_min_depth), int(bool(log_records)), 0)
+    _check(lib().zeta_search_batch(_ptr(pos), len(pos), ctypes.byref(o), _ptr(out)), "zeta_search_batch")
+    return out
 
 
 def playouts(seed, n, minply, maxply):

@@ -41,6 +41,8 @@ typedef uint64_t u64;
 #endif
 typedef uint32_t u32;
 typedef int32_t s32;
+typedef uint16_t u16;
+typedef uint8_t u8;
 
 constexpr u64 NOTA = 0xFEFEFEFEFEFEFEFEULL;
 constexpr u64 NOTH = 0x7F7F7F7F7F7F7F7FULL;

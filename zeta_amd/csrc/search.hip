@@ -1,0 +1,664 @@
+// search.hip -- batched fixed-depth alpha-beta + quiescence on gfx950.
+//
+// Replaces alphabeta_gpu (src/zeta.cl:852-2512) and its host driver
+// rootsearch (src/search.c:35-223), for many root positions per launch.
+//
+// Execution model: one wavefront = one worker; a persistent grid of workers
+// takes root positions from a queue and, once the queue is empty, joins
+// unfinished roots as ABDADA helpers (busy siblings deferred to a second pass,
+// src/zeta.cl:1648-1717).  The first worker to finish a root's last
+// iteration publishes its score; the others abandon that root.
+// Per worker, the depth-first stack lives in LDS: one Ply record per ply and a
+// move stack.  Node control is wave-uniform (every lane holds the same
+// values); lanes split the per-node work:
+//   * move generation: lane = square, each lane enumerates its piece's legal
+//     targets; a wave prefix sum compacts them into the LDS move stack;
+//   * move selection: wave arg-max over the node's remaining moves
+//     (score, then lowest slot, so runs are deterministic per worker);
+//   * evaluation: lane = square (src/zeta.cl:1482-1531), wave sum.
+//
+// Node semantics follow alphabeta_gpu with null move, LMR and IID off:
+// check / queen-promotion extension (src/zeta.cl:1330-1345), quiescence at
+// depth <= 0 with captures + e.p. only and stand pat when not in check,
+// mate = -INF+ply only outside quiescence, stalemate 0, in-path repetition,
+// fifty-move and bare-king draws below the root, MAXPLY cut, fail-hard
+// window propagation, root score = PV[0] semantics.
+//
+// Transposition table (2^k entries x 16 B in HBM, lockless: word0 =
+// key ^ word1): every entry supplies the move ordering's TT move; its score
+// is used to raise alpha only when it cannot change the root value: the
+// entry was stored at the same depth by a node whose half-move clock was 0
+// and the probing node's clock is 0 too (then neither subtree can see a
+// repetition above itself), and the key includes the exact castle squares.
+#include <string.h>
+
+#include "chess.h"
+#include "evaltab.h"
+#include "runtime.h"
+
+using namespace zc;
+
+namespace {
+
+constexpr int MAXPLY = 64;  // src/types.h:76
+constexpr s32 INF = 32000;
+constexpr s32 MATESCORE = 30000;
+constexpr int MSTACK = 1024;       // move-stack slots per worker (LDS)
+constexpr s32 USED = (s32)0x80000000;
+constexpr s32 TTMOVE_SCORE = 0x7FFFFFFF - 100;  // src/zeta.cl:2101
+constexpr s32 KILLER0_SCORE = 1400;             // src/zeta.cl:2082
+constexpr s32 KILLER1_SCORE = 1300;
+constexpr int ABDADA_MIN_DEPTH = 2;
+constexpr int POLL_EVERY = 64;
+
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte TT entry
+
+struct SearchArgs {
+  const u64 *roots;  // zeta_pos[n]
+  int nroots;
+  int depth;
+  int workers;
+  u64 *tt;           // 2 words per entry
+  u64 ttmask;        // entries - 1 (0 = no TT)
+  u32 *busy;
+  u32 busymask;
+  int *queue;
+  u32 *done;
+  int *nworkers;
+  s32 *score;
+  u32 *bestmove;
+  u64 *nodes;
+  s32 *depth_done;
+  u32 *err;
+  const u32 *abort_flag;
+  u64 max_nodes;  // per worker and root (0 = none)
+};
+
+struct Ply {
+  u64 b0, b1, b2, b3, cr, hash, phash;
+  s32 alpha, beta, depth;
+  u32 move, best, lastmove;
+  u16 mbeg, mcnt;
+  u8 stm, hmc, qs, kic, iter, ndefer, busy, searched;
+};
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }  // one wave per block
+
+__device__ __forceinline__ s32 wave_sum_i(s32 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ u64 wave_max_u64(u64 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    u64 w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_exscan_i(int v, int lane, int &total) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+__device__ __forceinline__ u64 mix64(u64 z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+// TT / busy-table key: the reference hash plus the exact castle squares
+__device__ __forceinline__ u64 tt_key(u64 hash, u64 cr) { return hash ^ mix64(cr & 0x9100000000000091ULL); }
+
+__device__ __forceinline__ bool is_mate(s32 v) { return (v > MATESCORE && v < INF) || (v < -MATESCORE && v > -INF); }
+
+__device__ __forceinline__ bool is_promo(u32 m) {
+  return m && ((mv_pfrom(m) >> 1) & 7) == PAWN && ((mv_pto(m) >> 1) & 7) == QUEEN;
+}
+
+struct Node {  // the current node, held uniformly in every lane's registers
+  Pos p;
+  u64 hash, phash;
+  s32 alpha, beta, depth;
+  u32 move, best, lastmove;
+  int mbeg, mcnt;
+  u32 hmc;
+  bool qs, kic;
+  int iter, ndefer, searched;
+  bool busy;
+};
+
+__device__ __forceinline__ void save(Ply &y, const Node &n) {
+  y.b0 = n.p.b0; y.b1 = n.p.b1; y.b2 = n.p.b2; y.b3 = n.p.b3; y.cr = n.p.cr;
+  y.hash = n.hash; y.phash = n.phash;
+  y.alpha = n.alpha; y.beta = n.beta; y.depth = n.depth;
+  y.move = n.move; y.best = n.best; y.lastmove = n.lastmove;
+  y.mbeg = (u16)n.mbeg; y.mcnt = (u16)n.mcnt;
+  y.stm = (u8)n.p.stm; y.hmc = (u8)n.hmc; y.qs = n.qs; y.kic = n.kic;
+  y.iter = (u8)n.iter; y.ndefer = (u8)n.ndefer; y.busy = n.busy; y.searched = (u8)n.searched;
+}
+__device__ __forceinline__ void load(const Ply &y, Node &n) {
+  n.p.b0 = y.b0; n.p.b1 = y.b1; n.p.b2 = y.b2; n.p.b3 = y.b3; n.p.cr = y.cr; n.p.stm = y.stm;
+  n.hash = y.hash; n.phash = y.phash;
+  n.alpha = y.alpha; n.beta = y.beta; n.depth = y.depth;
+  n.move = y.move; n.best = y.best; n.lastmove = y.lastmove;
+  n.mbeg = y.mbeg; n.mcnt = y.mcnt;
+  n.hmc = y.hmc; n.qs = y.qs; n.kic = y.kic;
+  n.iter = y.iter; n.ndefer = y.ndefer; n.busy = y.busy; n.searched = y.searched;
+}
+
+// child hash of the alpha-beta move-up (src/zeta.cl:2297-2359), incremental
+__device__ __forceinline__ void child_hash(const Node &n, u32 m, const Pos &c, u64 &hash, u64 &phash) {
+  const int f = (int)mv_from(m), t = (int)mv_to(m), x = (int)mv_cpt(m);
+  const u32 pf = mv_pfrom(m), pt = mv_pto(m), pc = mv_pcpt(m);
+  u64 ph = n.phash ^ piece_key(pf, f) ^ piece_key(pt, t);
+  if (((pc >> 1) & 7) != PNONE) ph ^= piece_key(pc, x);
+  if (((pf >> 1) & 7) == KING && (f - t == 2 || t - f == 2)) {
+    const u32 rook = (ROOK << 1) | (pf & 1);
+    const int rf = (f - t == 2) ? f - 4 : f + 3, rt = (f - t == 2) ? t + 1 : t - 1;
+    ph ^= piece_key(rook, rf) ^ piece_key(rook, rt);
+  }
+  u64 h = ph;
+  const u32 col = pf & 1;
+  const int rfr = col ? ((f >> 3) ^ 7) : (f >> 3), rto = col ? ((t >> 3) ^ 7) : (t >> 3);
+  if (((pf >> 1) & 7) == PAWN && rto - rfr == 2) h ^= rotl(ZOBRIST[16], t & 7);
+  if (c.stm) h ^= 1ULL;
+  h ^= hash_castle(c.cr);
+  hash = h;
+  phash = ph;
+}
+
+__device__ __forceinline__ u32 child_hmc(u32 hmc, u32 m) {
+  // src/zeta.cl:2362-2373 (8-bit counter)
+  const u32 pf = mv_pfrom(m);
+  const int f = (int)mv_from(m), t = (int)mv_to(m);
+  if (((pf >> 1) & 7) == PAWN || ((mv_pcpt(m) >> 1) & 7) != PNONE) return 0;
+  if (((pf >> 1) & 7) == KING && (f - t == 2 || t - f == 2)) return 0;
+  return (hmc + 1) & 0xFF;
+}
+
+enum { S_ENTER = 0, S_NEXT = 1, S_RETURN = 2 };
+
+// TT store when a node returns with a move that raised alpha (a lower bound
+// of the node's value); score kept as distance-to-mate from this node.
+__device__ __forceinline__ void tt_store(const SearchArgs &a, const Node &n, int sd, int lane) {
+  if (!a.ttmask || n.qs || !n.best || n.depth < 1) return;
+  s32 s = n.alpha;
+  const int plyn = sd - 1;
+  if (is_mate(s) && s > 0) s += plyn;
+  if (is_mate(s) && s < 0) s -= plyn;
+  const u64 key = tt_key(n.hash, n.p.cr);
+  const u64 w1 = (u64)n.best | ((u64)(u16)(int16_t)s << 32) | ((u64)(n.depth & 0xFF) << 48) |
+                 ((u64)(n.hmc == 0 ? 1 : 0) << 56) | (1ULL << 57);
+  if (lane == 0) {
+    const u64 w0 = key ^ w1;
+    u32x4 e;
+    e.x = (u32)w0;
+    e.y = (u32)(w0 >> 32);
+    e.z = (u32)w1;
+    e.w = (u32)(w1 >> 32);
+    *(u32x4 *)(a.tt + 2 * (key & a.ttmask)) = e;  // one 16-byte vector store
+  }
+}
+
+__global__ __launch_bounds__(64) void k_search(SearchArgs a) {
+  __shared__ Ply ply[MAXPLY + 2];
+  __shared__ u32 mmove[MSTACK];
+  __shared__ s32 mscore[MSTACK];
+  __shared__ u8 mflag[MSTACK];
+  __shared__ u32 killers[MAXPLY + 2][2];
+  __shared__ int sh_root;
+
+  const int lane = threadIdx.x;
+  const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
+
+  for (;;) {
+    // ---------------------------------------------------------------- pick a root
+    if (lane == 0) {
+      int r = atomicAdd(a.queue, 1);
+      if (r < a.nroots) {
+        atomicAdd(&a.nworkers[r], 1);
+      } else {
+        r = -1;
+        for (int k = 0; k < a.nroots && a.workers > 1; k++) {
+          const int rr = (hstart + k) % a.nroots;
+          if (__atomic_load_n(&a.done[rr], __ATOMIC_RELAXED)) continue;
+          if (__atomic_load_n(&a.nworkers[rr], __ATOMIC_RELAXED) >= a.workers) continue;
+          if (atomicAdd(&a.nworkers[rr], 1) < a.workers) {
+            r = rr;
+            break;
+          }
+          atomicSub(&a.nworkers[rr], 1);
+        }
+      }
+      if (*a.abort_flag) r = -1;
+      sh_root = r;
+    }
+    wsync();
+    const int r = sh_root;
+    wsync();
+    if (r < 0) return;
+
+    // ---------------------------------------------------------------- root set-up
+    const u64 *q = a.roots + (size_t)r * 8;
+    Pos root;
+    root.b0 = q[0]; root.b1 = q[1]; root.b2 = q[2]; root.b3 = q[3]; root.cr = q[4];
+    root.stm = (u32)(q[7] & 1u);
+    const u64 roothash = q[5];
+    const u32 roothmc = (u32)(q[6] & 0xFF);
+    for (int k = lane; k < (MAXPLY + 2) * 2; k += 64) killers[k >> 1][k & 1] = 0;
+    if (lane == 0) {
+      ply[0].qs = 0;
+      ply[0].mbeg = 0;
+      ply[0].mcnt = 0;
+    }
+    wsync();
+
+    u64 nodes = 0;
+    int poll = 0;
+    bool aborted = false, failed = false;
+    s32 final_score = 0;
+    u32 final_move = 0;
+
+    for (int id = 1; id <= a.depth && !aborted && !failed; id++) {
+      // root node
+      Node n;
+      n.p = root;
+      n.hash = roothash;
+      n.phash = hash_pieces(root);
+      n.alpha = -INF;
+      n.beta = INF;
+      n.depth = id;
+      n.lastmove = 0;
+      n.hmc = roothmc;
+      n.mbeg = 0;
+      n.mcnt = 0;
+      n.busy = false;
+      n.move = 0;
+      n.best = 0;
+      int sd = 1;
+      bool parent_qs = false;
+      s32 rootscore = 0;  // DRAWSCORE until a root move raises alpha (src/zeta.cl:986)
+      u32 rootmove = 0;
+      s32 v = 0;
+      int state = S_ENTER;
+
+      for (;;) {
+        if (state == S_ENTER) {
+          // ---------------- ENTER: analyse, extend, generate, terminal tests
+          if (++poll >= POLL_EVERY) {
+            poll = 0;
+            u32 d = 0;
+            if (lane == 0) {
+              d = __atomic_load_n(&a.done[r], __ATOMIC_RELAXED) | __atomic_load_n(a.abort_flag, __ATOMIC_RELAXED);
+              if (a.max_nodes && nodes > a.max_nodes) d = 1;
+            }
+            if (__shfl(d, 0, 64)) {
+              aborted = true;
+              break;
+            }
+          }
+          Legal L;
+          analyse(n.p, L);
+          n.kic = L.checkers != 0;
+          if (n.depth >= 0 && !parent_qs && (n.kic || is_promo(n.lastmove))) n.depth++;
+          n.qs = n.depth <= 0;
+          n.best = 0;
+          n.iter = 1;
+          n.ndefer = 0;
+          n.searched = 0;
+          // move generation, lane = square; wave prefix sum compacts the lists
+          u64 t = 0;
+          if ((L.own >> lane) & 1) t = piece_targets(n.p, L, lane, n.qs);
+          int total;
+          const int off = wave_exscan_i(popc(t), lane, total);
+          n.mcnt = total;
+          if (n.mbeg + total > MSTACK) {
+            if (lane == 0) atomicOr(a.err, 1u);
+            failed = true;
+            break;
+          }
+          {
+            int k = n.mbeg + off;
+            while (t) {
+              const int to = lsb(t);
+              t &= t - 1;
+              const u32 m = make_code(n.p, lane, to);
+              s32 sc = order_score(n.p.stm, m);
+              if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
+                if (m == killers[sd][0]) sc = KILLER0_SCORE;
+                else if (m == killers[sd][1]) sc = KILLER1_SCORE;
+              }
+              mmove[k] = m;
+              mscore[k] = sc;
+              mflag[k] = 0;
+              k++;
+            }
+          }
+          // terminal / draw tests in the reference's order (src/zeta.cl:1554-1609)
+          const int plyn = sd - 1;
+          bool draw = false;
+          if (total > 0 && !n.qs && sd > 1) {
+            // in-path repetition against plies plyn-2, plyn-4, .. >= plyn-hmc
+            bool hit = false;
+            const int nn = plyn - 2 - 2 * lane;
+            if (nn >= 0 && nn >= plyn - (int)n.hmc) hit = ply[nn + 1].hash == n.hash;
+            draw = __ballot(hit) != 0;
+          }
+          if (sd > 1 && n.hmc >= 100) draw = true;
+          if (sd > 1 && popc(L.occ) <= 2) draw = true;
+          const bool need_eval = !draw && ((n.qs && !n.kic) || (n.qs && total == 0) || (sd >= MAXPLY && total > 0));
+          s32 ev = 0;
+          if (need_eval) {
+            const s32 e = wave_sum_i(eval_square(n.p, lane)) + eval_bishop_pair(n.p);
+            ev = n.p.stm ? -e : e;
+          }
+          state = S_NEXT;
+          if (draw) {
+            v = 0;
+            state = S_RETURN;
+          } else if (!n.qs && total == 0) {
+            v = n.kic ? -INF + plyn : 0;
+            state = S_RETURN;
+          } else if (sd >= MAXPLY || total == 0) {
+            v = ev;
+            state = S_RETURN;
+          } else if (n.qs && !n.kic && ev >= n.beta) {
+            v = ev;  // stand pat, fail soft
+            state = S_RETURN;
+          } else {
+            if (n.qs && !n.kic && ev > n.alpha) n.alpha = ev;
+            // TT probe: move for ordering; score under the exact-depth rule
+            u32 ttmove = 0;
+            if (a.ttmask && !n.qs && sd > 1) {
+              const u64 key = tt_key(n.hash, n.p.cr);
+              const u32x4 e = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
+              const u64 w0 = ((u64)e.y << 32) | e.x, w1 = ((u64)e.w << 32) | e.z;
+              if ((w0 ^ w1) == key && ((w1 >> 57) & 1)) {
+                ttmove = (u32)w1;
+                s32 ts = (s32)(int16_t)(u16)(w1 >> 32);
+                const int d = (int)((w1 >> 48) & 0xFF);
+                if (is_mate(ts) && ts > 0) ts -= plyn;
+                if (is_mate(ts) && ts < 0) ts += plyn;
+                if (((w1 >> 56) & 1) && n.hmc == 0 && d == n.depth && ts > -INF && ts < INF && ts > n.alpha) {
+                  n.alpha = ts;
+                  if (n.alpha >= n.beta) {
+                    v = n.alpha;
+                    state = S_RETURN;
+                  }
+                }
+              }
+            }
+            if (state == S_NEXT && ttmove) {
+              for (int k = lane; k < total; k += 64)
+                if (mmove[n.mbeg + k] == ttmove) mscore[n.mbeg + k] = TTMOVE_SCORE;
+            }
+          }
+          wsync();
+          continue;
+        }
+
+        if (state == S_NEXT) {
+          // ---------------- NEXT: pick the best remaining move
+          u64 bestkey = 0;
+          for (int k = lane; k < n.mcnt; k += 64) {
+            const s32 sc = mscore[n.mbeg + k];
+            const bool ok = sc != USED && (n.iter == 1 ? mflag[n.mbeg + k] == 0 : mflag[n.mbeg + k] != 0);
+            if (ok) {
+              const u64 key = ((u64)((u32)sc ^ 0x80000000u) << 32) | (u64)(0xFFFFu - (u32)k) | (1ULL << 16);
+              bestkey = key > bestkey ? key : bestkey;
+            }
+          }
+          bestkey = wave_max_u64(bestkey);
+          if (bestkey == 0) {
+            if (n.iter == 1 && n.ndefer > 0) {
+              n.iter = 2;
+              continue;
+            }
+            v = n.alpha;
+            tt_store(a, n, sd, lane);
+            state = S_RETURN;
+            continue;
+          }
+          const int k = n.mbeg + (int)(0xFFFFu - (u32)(bestkey & 0xFFFF));
+          const u32 m = mmove[k];
+          const s32 sk = mscore[k];
+          wsync();
+          if (lane == 0) mscore[k] = USED;
+          const Pos c = apply(n.p, m, false);
+          u64 ch, cph;
+          child_hash(n, m, c, ch, cph);
+          // ABDADA: defer a sibling that another worker is already searching
+          bool cbusy = false;
+          if (a.workers > 1 && n.depth - 1 >= ABDADA_MIN_DEPTH) {
+            const u32 bidx = (u32)(tt_key(ch, c.cr) >> 40) & a.busymask;
+            u32 b = 0;
+            if (lane == 0) b = __atomic_load_n(&a.busy[bidx], __ATOMIC_RELAXED);
+            b = __shfl(b, 0, 64);
+            if (n.iter == 1 && n.searched > 0 && b) {
+              if (lane == 0) {
+                mscore[k] = sk;
+                mflag[k] = 1;
+              }
+              n.ndefer++;
+              wsync();
+              continue;
+            }
+            if (lane == 0) atomicAdd(&a.busy[bidx], 1u);
+            cbusy = true;
+          }
+          if (n.iter == 1) n.searched++;
+          nodes++;
+          n.move = m;
+          if (lane == 0) save(ply[sd], n);
+          // descend
+          parent_qs = n.qs;
+          Node cn;
+          cn.p = c;
+          cn.hash = ch;
+          cn.phash = cph;
+          cn.alpha = -n.beta;
+          cn.beta = -n.alpha;
+          cn.depth = n.depth - 1;
+          cn.lastmove = m;
+          cn.hmc = child_hmc(n.hmc, m);
+          cn.mbeg = n.mbeg + n.mcnt;
+          cn.mcnt = 0;
+          cn.busy = cbusy;
+          cn.move = 0;
+          cn.best = 0;
+          cn.qs = false;
+          cn.kic = false;
+          cn.iter = 1;
+          cn.ndefer = 0;
+          cn.searched = 0;
+          n = cn;
+          sd++;
+          wsync();
+          state = S_ENTER;
+          continue;
+        }
+
+        // ---------------- RETURN v from node sd
+        if (n.busy) {
+          const u32 bidx = (u32)(tt_key(n.hash, n.p.cr) >> 40) & a.busymask;
+          if (lane == 0) atomicSub(&a.busy[bidx], 1u);
+        }
+        if (sd == 1) break;
+        sd--;
+        wsync();
+        load(ply[sd], n);
+        const s32 sc = -v;
+        if (sc > n.alpha) {
+          n.alpha = sc;
+          n.best = n.move;
+          if (sd == 1) {
+            rootscore = sc;
+            rootmove = n.move;
+          }
+        }
+        if (n.alpha >= n.beta) {
+          if (((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0 && killers[sd][0] != n.move) {
+            killers[sd][1] = killers[sd][0];
+            killers[sd][0] = n.move;
+          }
+          v = n.alpha;
+          tt_store(a, n, sd, lane);
+          state = S_RETURN;
+        } else {
+          state = S_NEXT;
+        }
+        wsync();
+      }
+
+      if (aborted || failed) {
+        // release the ABDADA marks still held along the path
+        if (n.busy) {
+          const u32 bidx = (u32)(tt_key(n.hash, n.p.cr) >> 40) & a.busymask;
+          if (lane == 0) atomicSub(&a.busy[bidx], 1u);
+        }
+        wsync();
+        for (int k = 2; k < sd; k++) {
+          if (ply[k].busy) {
+            const u32 bidx = (u32)(tt_key(ply[k].hash, ply[k].cr) >> 40) & a.busymask;
+            if (lane == 0) atomicSub(&a.busy[bidx], 1u);
+          }
+        }
+        break;
+      }
+      final_score = (rootscore == INF || rootscore == -INF) ? 0 : rootscore;  // src/search.c:137
+      final_move = rootmove;
+      if (lane == 0) atomicMax(&a.depth_done[r], id);
+    }
+
+    // ---------------------------------------------------------------- publish
+    if (lane == 0) {
+      atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
+      if (!aborted && !failed && atomicCAS(&a.done[r], 0u, 1u) == 0u) {
+        a.score[r] = final_score;
+        a.bestmove[r] = final_move;
+      }
+      if (failed) atomicCAS(&a.done[r], 0u, 2u);
+      atomicSub(&a.nworkers[r], 1);
+    }
+    wsync();
+  }
+}
+
+__global__ void k_clear_u64(u64 *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+struct SearchState {
+  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf;
+  u64 tt_entries = 0;
+};
+SearchState g_ss;
+
+inline unsigned nblocks(size_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+void search_release() {
+  zr::release(g_ss.tt); zr::release(g_ss.busy); zr::release(g_ss.roots); zr::release(g_ss.queue);
+  zr::release(g_ss.done); zr::release(g_ss.nworkers); zr::release(g_ss.score); zr::release(g_ss.bestmove);
+  zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err); zr::release(g_ss.abortf);
+  g_ss.tt_entries = 0;
+}
+
+extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, zeta_result *res) {
+  int r = zr::require_up();
+  if (r) return r;
+  if (n < 0 || (n > 0 && (!pos || !res)) || !opts) return zr::fail(ZETA_EINVAL, "search: bad arguments");
+  if (opts->depth < 1 || opts->depth > 40) return zr::fail(ZETA_EINVAL, "search: depth %d out of range", opts->depth);
+  if (n == 0) return ZETA_OK;
+  const int workers = opts->workers < 1 ? 1 : (opts->workers > 64 ? 64 : opts->workers);
+  hipStream_t st = zr::ctx().stream;
+  const int tt_log2 = opts->tt_log2 < 0 ? zr::ctx().cfg.tt_log2 : opts->tt_log2;
+  const u64 tt_entries = tt_log2 > 0 ? (1ULL << tt_log2) : 0;
+  if (tt_log2 > 34) return zr::fail(ZETA_EINVAL, "search: tt_log2 %d too large", tt_log2);
+  const u32 busy_entries = 1u << 20;
+  if ((r = zr::ensure(g_ss.roots, (size_t)n * sizeof(zeta_pos))) || (r = zr::ensure(g_ss.queue, 64)) ||
+      (r = zr::ensure(g_ss.done, (size_t)n * 4)) || (r = zr::ensure(g_ss.nworkers, (size_t)n * 4)) ||
+      (r = zr::ensure(g_ss.score, (size_t)n * 4)) || (r = zr::ensure(g_ss.bestmove, (size_t)n * 4)) ||
+      (r = zr::ensure(g_ss.nodes, (size_t)n * 8)) || (r = zr::ensure(g_ss.depth_done, (size_t)n * 4)) ||
+      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.abortf, 64)) ||
+      (r = zr::ensure(g_ss.busy, (size_t)busy_entries * 4)))
+    return r;
+  if (tt_entries) {
+    if ((r = zr::ensure(g_ss.tt, tt_entries * 16))) return r;
+    if (g_ss.tt_entries != tt_entries || opts->clear_tt) {
+      ZLAUNCH(zr::K_SEARCH, 0, k_clear_u64, 4096, 256, 0, (u64 *)g_ss.tt.p, (size_t)(tt_entries * 2));
+      g_ss.tt_entries = tt_entries;
+    }
+  }
+  ZCHECK(hipMemcpyAsync(g_ss.roots.p, pos, (size_t)n * sizeof(zeta_pos), hipMemcpyHostToDevice, st));
+  ZCHECK(hipMemsetAsync(g_ss.queue.p, 0, 64, st));
+  ZCHECK(hipMemsetAsync(g_ss.done.p, 0, (size_t)n * 4, st));
+  ZCHECK(hipMemsetAsync(g_ss.nworkers.p, 0, (size_t)n * 4, st));
+  ZCHECK(hipMemsetAsync(g_ss.score.p, 0, (size_t)n * 4, st));
+  ZCHECK(hipMemsetAsync(g_ss.bestmove.p, 0, (size_t)n * 4, st));
+  ZCHECK(hipMemsetAsync(g_ss.nodes.p, 0, (size_t)n * 8, st));
+  ZCHECK(hipMemsetAsync(g_ss.depth_done.p, 0, (size_t)n * 4, st));
+  ZCHECK(hipMemsetAsync(g_ss.err.p, 0, 64, st));
+  ZCHECK(hipMemsetAsync(g_ss.abortf.p, 0, 64, st));
+  ZCHECK(hipMemsetAsync(g_ss.busy.p, 0, (size_t)busy_entries * 4, st));
+
+  int per_cu = 0;
+  ZCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search, 64, 0));
+  if (per_cu < 1) per_cu = 1;
+  const u64 cap = (u64)per_cu * (u64)zr::ctx().cu_count;
+  u64 want = (u64)n * (u64)workers;
+  const unsigned grid = (unsigned)(want < cap ? want : cap);
+
+  SearchArgs sa;
+  sa.roots = (const u64 *)g_ss.roots.p;
+  sa.nroots = n;
+  sa.depth = opts->depth;
+  sa.workers = workers;
+  sa.tt = (u64 *)g_ss.tt.p;
+  sa.ttmask = tt_entries ? tt_entries - 1 : 0;
+  sa.busy = (u32 *)g_ss.busy.p;
+  sa.busymask = busy_entries - 1;
+  sa.queue = (int *)g_ss.queue.p;
+  sa.done = (u32 *)g_ss.done.p;
+  sa.nworkers = (int *)g_ss.nworkers.p;
+  sa.score = (s32 *)g_ss.score.p;
+  sa.bestmove = (u32 *)g_ss.bestmove.p;
+  sa.nodes = (u64 *)g_ss.nodes.p;
+  sa.depth_done = (s32 *)g_ss.depth_done.p;
+  sa.err = (u32 *)g_ss.err.p;
+  sa.abort_flag = (const u32 *)g_ss.abortf.p;
+  sa.max_nodes = opts->max_nodes;
+  ZLAUNCH(zr::K_SEARCH, (u64)n, k_search, grid, 64, 0, sa);
+
+  std::vector<s32> score(n), dd(n);
+  std::vector<u32> bm(n), done(n);
+  std::vector<u64> nodes(n);
+  u32 err = 0;
+  ZCHECK(hipMemcpyAsync(score.data(), g_ss.score.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipMemcpyAsync(bm.data(), g_ss.bestmove.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipMemcpyAsync(done.data(), g_ss.done.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipMemcpyAsync(nodes.data(), g_ss.nodes.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipMemcpyAsync(dd.data(), g_ss.depth_done.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipMemcpyAsync(&err, g_ss.err.p, 4, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipStreamSynchronize(st));
+  zr::time_collect();
+  for (int i = 0; i < n; i++) {
+    res[i].score = score[i];
+    res[i].bestmove = bm[i];
+    res[i].nodes = nodes[i];
+    res[i].depth = dd[i];
+    res[i].status = done[i] == 1 ? 0 : (done[i] == 2 ? ZETA_ENOMEM : ZETA_ESTATE);
+  }
+  if (err & 1u) return zr::fail(ZETA_ENOMEM, "search: move stack overflow (more than %d moves on one path)", MSTACK);
+  for (int i = 0; i < n; i++)
+    if (done[i] != 1 && opts->max_nodes == 0) return zr::fail(ZETA_ESTATE, "search: root %d did not complete", i);
+  return ZETA_OK;
+}
