@@ -110,6 +110,10 @@ typedef struct {
 } zeta_result;
 
 int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, zeta_result *res);
+/* Counters of the last zeta_search_batch: [0] TT probes, [1] TT key hits,
+ * [2] TT scores used, [3] TT stores, [4] ABDADA deferrals, [5] MAXPLY cuts.
+ * Each probe / store moves one 16-byte entry. */
+int zeta_search_counters(uint64_t *out, int n);
 
 /* ------------------------------------------------------------ device memory
  * Device buffers for callers that keep inputs resident on the GPU (the

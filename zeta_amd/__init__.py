@@ -74,6 +74,7 @@ def lib():
         L.zeta_stats_get.argtypes = [ctypes.POINTER(_Stats)]
         L.zeta_set_timing.argtypes = [i32]
         L.zeta_search_batch.argtypes = [vp, i32, ctypes.POINTER(SearchOpts), vp]
+        L.zeta_search_counters.argtypes = [vp, i32]
         L.zeta_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         L.zeta_dev_free.argtypes = [vp]
         L.zeta_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
@@ -82,7 +83,7 @@ def lib():
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
                   "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
                   "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize",
-                  "zeta_search_batch"):
+                  "zeta_search_batch", "zeta_search_counters"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -164,6 +165,16 @@ def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_ml
                    int(bool(use_mlp)), int(mlp_min_depth), int(bool(log_records)), 0)
     _check(lib().zeta_search_batch(_ptr(pos), len(pos), ctypes.byref(o), _ptr(out)), "zeta_search_batch")
     return out
+
+
+def search_counters():
+    """TT probes, TT hits, TT scores used, TT stores, ABDADA deferrals, MAXPLY
+    cuts of the last search batch."""
+    ensure_init()
+    out = np.zeros(8, dtype=np.uint64)
+    _check(lib().zeta_search_counters(_ptr(out), 8), "zeta_search_counters")
+    keys = ("tt_probes", "tt_hits", "tt_cuts", "tt_stores", "abdada_defers", "maxply_cuts")
+    return {k: int(out[i]) for i, k in enumerate(keys)}
 
 
 def playouts(seed, n, minply, maxply):

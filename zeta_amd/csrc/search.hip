@@ -72,7 +72,11 @@ struct SearchArgs {
   u32 *err;
   const u32 *abort_flag;
   u64 max_nodes;  // per worker and root (0 = none)
+  u64 *ctr;       // batch counters, see zeta_search_counters
 };
+
+// batch counters (device, summed per worker at the end of each root)
+enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_NCTR = 8 };
 
 struct Ply {
   u64 b0, b1, b2, b3, cr, hash, phash;
@@ -187,8 +191,8 @@ enum { S_ENTER = 0, S_NEXT = 1, S_RETURN = 2 };
 
 // TT store when a node returns with a move that raised alpha (a lower bound
 // of the node's value); score kept as distance-to-mate from this node.
-__device__ __forceinline__ void tt_store(const SearchArgs &a, const Node &n, int sd, int lane) {
-  if (!a.ttmask || n.qs || !n.best || n.depth < 1) return;
+__device__ __forceinline__ u32 tt_store(const SearchArgs &a, const Node &n, int sd, int lane) {
+  if (!a.ttmask || n.qs || !n.best || n.depth < 1) return 0;
   s32 s = n.alpha;
   const int plyn = sd - 1;
   if (is_mate(s) && s > 0) s += plyn;
@@ -205,6 +209,7 @@ __device__ __forceinline__ void tt_store(const SearchArgs &a, const Node &n, int
     e.w = (u32)(w1 >> 32);
     *(u32x4 *)(a.tt + 2 * (key & a.ttmask)) = e;  // one 16-byte vector store
   }
+  return 1;
 }
 
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
@@ -261,6 +266,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     wsync();
 
     u64 nodes = 0;
+    u32 c_probe = 0, c_hit = 0, c_cut = 0, c_store = 0, c_defer = 0, c_maxply = 0;
     int poll = 0;
     bool aborted = false, failed = false;
     s32 final_score = 0;
@@ -367,6 +373,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             v = n.kic ? -INF + plyn : 0;
             state = S_RETURN;
           } else if (sd >= MAXPLY || total == 0) {
+            if (sd >= MAXPLY) c_maxply++;
             v = ev;
             state = S_RETURN;
           } else if (n.qs && !n.kic && ev >= n.beta) {
@@ -380,7 +387,9 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
               const u64 key = tt_key(n.hash, n.p.cr);
               const u32x4 e = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
               const u64 w0 = ((u64)e.y << 32) | e.x, w1 = ((u64)e.w << 32) | e.z;
+              c_probe++;
               if ((w0 ^ w1) == key && ((w1 >> 57) & 1)) {
+                c_hit++;
                 ttmove = (u32)w1;
                 s32 ts = (s32)(int16_t)(u16)(w1 >> 32);
                 const int d = (int)((w1 >> 48) & 0xFF);
@@ -388,6 +397,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 if (is_mate(ts) && ts < 0) ts += plyn;
                 if (((w1 >> 56) & 1) && n.hmc == 0 && d == n.depth && ts > -INF && ts < INF && ts > n.alpha) {
                   n.alpha = ts;
+                  c_cut++;
                   if (n.alpha >= n.beta) {
                     v = n.alpha;
                     state = S_RETURN;
@@ -422,7 +432,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
               continue;
             }
             v = n.alpha;
-            tt_store(a, n, sd, lane);
+            c_store += tt_store(a, n, sd, lane);
             state = S_RETURN;
             continue;
           }
@@ -447,6 +457,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
                 mflag[k] = 1;
               }
               n.ndefer++;
+              c_defer++;
               wsync();
               continue;
             }
@@ -509,7 +520,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             killers[sd][0] = n.move;
           }
           v = n.alpha;
-          tt_store(a, n, sd, lane);
+          c_store += tt_store(a, n, sd, lane);
           state = S_RETURN;
         } else {
           state = S_NEXT;
@@ -540,6 +551,12 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     // ---------------------------------------------------------------- publish
     if (lane == 0) {
       atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
+      atomicAdd((unsigned long long *)&a.ctr[C_TT_PROBE], (unsigned long long)c_probe);
+      atomicAdd((unsigned long long *)&a.ctr[C_TT_HIT], (unsigned long long)c_hit);
+      atomicAdd((unsigned long long *)&a.ctr[C_TT_CUT], (unsigned long long)c_cut);
+      atomicAdd((unsigned long long *)&a.ctr[C_TT_STORE], (unsigned long long)c_store);
+      atomicAdd((unsigned long long *)&a.ctr[C_DEFER], (unsigned long long)c_defer);
+      atomicAdd((unsigned long long *)&a.ctr[C_MAXPLY], (unsigned long long)c_maxply);
       if (!aborted && !failed && atomicCAS(&a.done[r], 0u, 1u) == 0u) {
         a.score[r] = final_score;
         a.bestmove[r] = final_move;
@@ -556,8 +573,9 @@ __global__ void k_clear_u64(u64 *p, size_t n) {
 }
 
 struct SearchState {
-  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf;
+  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf, ctr;
   u64 tt_entries = 0;
+  u64 last_ctr[C_NCTR] = {0};
 };
 SearchState g_ss;
 
@@ -569,6 +587,7 @@ void search_release() {
   zr::release(g_ss.tt); zr::release(g_ss.busy); zr::release(g_ss.roots); zr::release(g_ss.queue);
   zr::release(g_ss.done); zr::release(g_ss.nworkers); zr::release(g_ss.score); zr::release(g_ss.bestmove);
   zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err); zr::release(g_ss.abortf);
+  zr::release(g_ss.ctr);
   g_ss.tt_entries = 0;
 }
 
@@ -588,7 +607,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
       (r = zr::ensure(g_ss.done, (size_t)n * 4)) || (r = zr::ensure(g_ss.nworkers, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.score, (size_t)n * 4)) || (r = zr::ensure(g_ss.bestmove, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.nodes, (size_t)n * 8)) || (r = zr::ensure(g_ss.depth_done, (size_t)n * 4)) ||
-      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.abortf, 64)) ||
+      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.abortf, 64)) || (r = zr::ensure(g_ss.ctr, 64)) ||
       (r = zr::ensure(g_ss.busy, (size_t)busy_entries * 4)))
     return r;
   if (tt_entries) {
@@ -608,6 +627,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   ZCHECK(hipMemsetAsync(g_ss.depth_done.p, 0, (size_t)n * 4, st));
   ZCHECK(hipMemsetAsync(g_ss.err.p, 0, 64, st));
   ZCHECK(hipMemsetAsync(g_ss.abortf.p, 0, 64, st));
+  ZCHECK(hipMemsetAsync(g_ss.ctr.p, 0, 64, st));
   ZCHECK(hipMemsetAsync(g_ss.busy.p, 0, (size_t)busy_entries * 4, st));
 
   int per_cu = 0;
@@ -636,6 +656,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.err = (u32 *)g_ss.err.p;
   sa.abort_flag = (const u32 *)g_ss.abortf.p;
   sa.max_nodes = opts->max_nodes;
+  sa.ctr = (u64 *)g_ss.ctr.p;
   ZLAUNCH(zr::K_SEARCH, (u64)n, k_search, grid, 64, 0, sa);
 
   std::vector<s32> score(n), dd(n);
@@ -648,6 +669,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   ZCHECK(hipMemcpyAsync(nodes.data(), g_ss.nodes.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
   ZCHECK(hipMemcpyAsync(dd.data(), g_ss.depth_done.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   ZCHECK(hipMemcpyAsync(&err, g_ss.err.p, 4, hipMemcpyDeviceToHost, st));
+  ZCHECK(hipMemcpyAsync(g_ss.last_ctr, g_ss.ctr.p, sizeof(g_ss.last_ctr), hipMemcpyDeviceToHost, st));
   ZCHECK(hipStreamSynchronize(st));
   zr::time_collect();
   for (int i = 0; i < n; i++) {
@@ -660,5 +682,13 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   if (err & 1u) return zr::fail(ZETA_ENOMEM, "search: move stack overflow (more than %d moves on one path)", MSTACK);
   for (int i = 0; i < n; i++)
     if (done[i] != 1 && opts->max_nodes == 0) return zr::fail(ZETA_ESTATE, "search: root %d did not complete", i);
+  return ZETA_OK;
+}
+
+extern "C" int zeta_search_counters(uint64_t *out, int n) {
+  int r = zr::require_up();
+  if (r) return r;
+  if (!out || n < 0) return zr::fail(ZETA_EINVAL, "zeta_search_counters: bad arguments");
+  for (int i = 0; i < n && i < C_NCTR; i++) out[i] = g_ss.last_ctr[i];
   return ZETA_OK;
 }
