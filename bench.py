@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--positions", type=int, default=4096)
     ap.add_argument("--nobulk", action="store_true", help="visit every leaf (no bulk counting)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-search", action="store_true", help="skip the fixed-depth search leg")
+    ap.add_argument("--search-depth", type=int, default=6)
+    ap.add_argument("--search-workers", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (positions shard across GPUs with a "
@@ -108,6 +111,33 @@ def main():
                 "kernel": "perft_leaf", "avg_launch_ms": round(leaf[1] / leaf[0], 4),
                 "items_per_launch": int(per_launch_items), "ops64_per_item": ops[key]}
 
+    # config 1: startpos perft(5), one root (the reference's own perft command)
+    start = np.zeros(1, dtype=zeta_amd.POS_DTYPE)
+    start["b"][0] = [0xFFFF000000000000, 0x91FF00000000FF91, 0x5A0000000000005A, 0xAD000000000000AD,
+                     0x6EFFFFFFFFFFFF6E, 0, 0]
+    zeta_amd.perft(start, 5)
+    t1 = time.perf_counter()
+    c1 = int(zeta_amd.perft(start, 5)[0])
+    c1_ms = (time.perf_counter() - t1) * 1e3
+
+    # config 3: fixed-depth alpha-beta + quiescence, 1024 positions (seed 3,
+    # 20-60 plies), iterative deepening 1..depth, TT 2^26 x 16 B, ABDADA helpers
+    search = None
+    if not args.no_search:
+        spos = zeta_amd.playouts(3, 1024, 20, 60)
+        t2 = time.perf_counter()
+        res = zeta_amd.search(spos, args.search_depth, workers=args.search_workers, tt_log2=26, clear_tt=True)
+        ttd = time.perf_counter() - t2
+        ctr = zeta_amd.search_counters()
+        nodes = int(res["nodes"].sum())
+        tt_bytes = 16 * (ctr["tt_probes"] + ctr["tt_stores"])
+        search = {"workload": "1024 playout positions seed 3, 20-60 plies, depth %d + quiescence, TT 2^26 x 16 B, "
+                              "%d workers max per root" % (args.search_depth, args.search_workers),
+                  "time_to_depth_s": round(ttd, 3), "nodes": nodes, "nodes_per_s": nodes / ttd,
+                  "score_checksum": int(res["score"].astype(np.int64).sum()),
+                  "tt_GBps": round(tt_bytes / ttd / 1e9, 3), "tt_frac_of_hbm": round(tt_bytes / ttd / HBM_PEAK, 6),
+                  **ctr}
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
@@ -121,7 +151,10 @@ def main():
                       "positions": args.positions, "depth": args.depth, "leaves_per_step": leaves,
                       "bulk_counting": not args.nobulk},
            "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in kst.items() if v[0]},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu,
+           "config1": {"workload": "startpos perft(5), one root", "leaves": c1, "ms": round(c1_ms, 3),
+                       "leaves_per_s": c1 / (c1_ms / 1e3)},
+           "search": search}
     print(json.dumps(out))
 
 

@@ -430,6 +430,31 @@ ZD Pos apply(const Pos &p, u32 m, bool rook_flag) {
   return c;
 }
 
+// Inverse of apply(): the parent board from the child board and the move;
+// the parent's moved flags come from the caller (they are not invertible).
+ZD Pos undo(const Pos &c, u32 m, u64 parent_cr) {
+  Pos p = c;
+  const int f = (int)mv_from(m), t = (int)mv_to(m), x = (int)mv_cpt(m);
+  const u32 pf = mv_pfrom(m), pc = mv_pcpt(m);
+  const bool king = ((pf >> 1) & 7) == KING;
+  const bool qside = king && f - t == 2, kside = king && t - f == 2;
+  u64 keep = ~bit(t);
+  if (qside) keep &= ~bit(t + 1);
+  if (kside) keep &= ~bit(t - 1);
+  p.b0 &= keep;
+  p.b1 &= keep;
+  p.b2 &= keep;
+  p.b3 &= keep;
+  put_piece(p, f, pf);
+  if (((pc >> 1) & 7) != PNONE) put_piece(p, x, pc);
+  const u32 rook = (ROOK << 1) | (pf & 1);
+  if (qside) put_piece(p, f - 4, rook);
+  if (kside) put_piece(p, f + 3, rook);
+  p.cr = parent_cr;
+  p.stm = c.stm ^ 1u;
+  return p;
+}
+
 // ---- Zobrist hashing, rotate-left scheme: key(piece) rotated left by the
 // square (src/bitboard.c:461-508, src/zeta.cl:2297-2359).  12 piece keys
 // (colour*6 + type-1), 4 castle keys, 1 e.p. key rotated by the file; the

@@ -43,10 +43,10 @@ namespace {
 constexpr int MAXPLY = 64;  // src/types.h:76
 constexpr s32 INF = 32000;
 constexpr s32 MATESCORE = 30000;
-constexpr int MSTACK = 1024;       // move-stack slots per worker (LDS)
-constexpr s32 USED = (s32)0x80000000;
-constexpr s32 TTMOVE_SCORE = 0x7FFFFFFF - 100;  // src/zeta.cl:2101
-constexpr s32 KILLER0_SCORE = 1400;             // src/zeta.cl:2082
+constexpr int MSTACK = 1024;  // move-stack slots per worker (LDS)
+constexpr s32 USED = -32768;  // 16-bit move-ordering scores
+constexpr s32 TTMOVE_SCORE = 32000;  // above every other score (src/zeta.cl:2101)
+constexpr s32 KILLER0_SCORE = 1400;  // src/zeta.cl:2082
 constexpr s32 KILLER1_SCORE = 1300;
 constexpr int ABDADA_MIN_DEPTH = 2;
 constexpr int POLL_EVERY = 64;
@@ -57,7 +57,7 @@ struct SearchArgs {
   const u64 *roots;  // zeta_pos[n]
   int nroots;
   int depth;
-  int workers;
+  int workers;       // at most this many workers on one root
   u64 *tt;           // 2 words per entry
   u64 ttmask;        // entries - 1 (0 = no TT)
   u32 *busy;
@@ -78,8 +78,10 @@ struct SearchArgs {
 // batch counters (device, summed per worker at the end of each root)
 enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_NCTR = 8 };
 
+// One ply of a worker's stack in LDS (64 B).  The board itself is not kept:
+// a parent board is recovered with undo() when a child returns.
 struct Ply {
-  u64 b0, b1, b2, b3, cr, hash, phash;
+  u64 cr, hash, phash;
   s32 alpha, beta, depth;
   u32 move, best, lastmove;
   u16 mbeg, mcnt;
@@ -88,27 +90,33 @@ struct Ply {
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }  // one wave per block
 
+// Wave-uniform values: readfirstlane puts them in SGPRs, so the node logic
+// (position analysis, window and TT bookkeeping) runs on the scalar unit.
+__device__ __forceinline__ u32 uf(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ s32 uf(s32 v) { return (s32)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ u64 uf(u64 v) { return ((u64)uf((u32)(v >> 32)) << 32) | (u64)uf((u32)v); }
+
 __device__ __forceinline__ s32 wave_sum_i(s32 v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  return uf(v);
 }
-__device__ __forceinline__ u64 wave_max_u64(u64 v) {
+__device__ __forceinline__ u32 wave_max_u32(u32 v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    u64 w = __shfl_xor(v, o, 64);
+    const u32 w = (u32)__shfl_xor((int)v, o, 64);
     v = w > v ? w : v;
   }
-  return v;
+  return uf(v);
 }
 __device__ __forceinline__ int wave_exscan_i(int v, int lane, int &total) {
   int x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(x, o, 64);
+    const int y = __shfl_up(x, o, 64);
     if (lane >= o) x += y;
   }
-  total = __shfl(x, 63, 64);
+  total = uf(__shfl(x, 63, 64));
   return x - v;
 }
 
@@ -126,7 +134,7 @@ __device__ __forceinline__ bool is_promo(u32 m) {
   return m && ((mv_pfrom(m) >> 1) & 7) == PAWN && ((mv_pto(m) >> 1) & 7) == QUEEN;
 }
 
-struct Node {  // the current node, held uniformly in every lane's registers
+struct Node {  // the current node (wave-uniform)
   Pos p;
   u64 hash, phash;
   s32 alpha, beta, depth;
@@ -139,22 +147,46 @@ struct Node {  // the current node, held uniformly in every lane's registers
 };
 
 __device__ __forceinline__ void save(Ply &y, const Node &n) {
-  y.b0 = n.p.b0; y.b1 = n.p.b1; y.b2 = n.p.b2; y.b3 = n.p.b3; y.cr = n.p.cr;
-  y.hash = n.hash; y.phash = n.phash;
-  y.alpha = n.alpha; y.beta = n.beta; y.depth = n.depth;
-  y.move = n.move; y.best = n.best; y.lastmove = n.lastmove;
-  y.mbeg = (u16)n.mbeg; y.mcnt = (u16)n.mcnt;
-  y.stm = (u8)n.p.stm; y.hmc = (u8)n.hmc; y.qs = n.qs; y.kic = n.kic;
-  y.iter = (u8)n.iter; y.ndefer = (u8)n.ndefer; y.busy = n.busy; y.searched = (u8)n.searched;
+  y.cr = n.p.cr;
+  y.hash = n.hash;
+  y.phash = n.phash;
+  y.alpha = n.alpha;
+  y.beta = n.beta;
+  y.depth = n.depth;
+  y.move = n.move;
+  y.best = n.best;
+  y.lastmove = n.lastmove;
+  y.mbeg = (u16)n.mbeg;
+  y.mcnt = (u16)n.mcnt;
+  y.stm = (u8)n.p.stm;
+  y.hmc = (u8)n.hmc;
+  y.qs = n.qs;
+  y.kic = n.kic;
+  y.iter = (u8)n.iter;
+  y.ndefer = (u8)n.ndefer;
+  y.busy = n.busy;
+  y.searched = (u8)n.searched;
 }
-__device__ __forceinline__ void load(const Ply &y, Node &n) {
-  n.p.b0 = y.b0; n.p.b1 = y.b1; n.p.b2 = y.b2; n.p.b3 = y.b3; n.p.cr = y.cr; n.p.stm = y.stm;
-  n.hash = y.hash; n.phash = y.phash;
-  n.alpha = y.alpha; n.beta = y.beta; n.depth = y.depth;
-  n.move = y.move; n.best = y.best; n.lastmove = y.lastmove;
-  n.mbeg = y.mbeg; n.mcnt = y.mcnt;
-  n.hmc = y.hmc; n.qs = y.qs; n.kic = y.kic;
-  n.iter = y.iter; n.ndefer = y.ndefer; n.busy = y.busy; n.searched = y.searched;
+// parent node from its LDS record and the returning child's board
+__device__ __forceinline__ void restore(const Ply &y, const Pos &child, Node &n) {
+  n.hash = uf(y.hash);
+  n.phash = uf(y.phash);
+  n.alpha = uf(y.alpha);
+  n.beta = uf(y.beta);
+  n.depth = uf(y.depth);
+  n.move = uf(y.move);
+  n.best = uf(y.best);
+  n.lastmove = uf(y.lastmove);
+  n.mbeg = (int)uf((u32)y.mbeg);
+  n.mcnt = (int)uf((u32)y.mcnt);
+  n.hmc = uf((u32)y.hmc);
+  n.qs = uf((u32)y.qs) != 0;
+  n.kic = uf((u32)y.kic) != 0;
+  n.iter = (int)uf((u32)y.iter);
+  n.ndefer = (int)uf((u32)y.ndefer);
+  n.busy = uf((u32)y.busy) != 0;
+  n.searched = (int)uf((u32)y.searched);
+  n.p = undo(child, n.move, uf(y.cr));
 }
 
 // child hash of the alpha-beta move-up (src/zeta.cl:2297-2359), incremental
@@ -212,57 +244,67 @@ __device__ __forceinline__ u32 tt_store(const SearchArgs &a, const Node &n, int 
   return 1;
 }
 
+__device__ __forceinline__ void busy_release(const SearchArgs &a, u64 hash, u64 cr, int lane) {
+  const u32 bidx = (u32)(tt_key(hash, cr) >> 40) & a.busymask;
+  if (lane == 0) atomicSub(&a.busy[bidx], 1u);
+}
+
+// next root for this worker: the queue first, then the unfinished root with
+// the fewest workers (ABDADA helper), at most a.workers per root
+__device__ int pick_root(const SearchArgs &a, int hstart) {
+  int r = atomicAdd(a.queue, 1);
+  if (r < a.nroots) {
+    atomicAdd(&a.nworkers[r], 1);
+    return r;
+  }
+  if (a.workers <= 1) return -1;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    int best = -1, bw = a.workers;
+    for (int k = 0; k < a.nroots; k++) {
+      const int rr = (hstart + k) % a.nroots;
+      if (__atomic_load_n(&a.done[rr], __ATOMIC_RELAXED)) continue;
+      const int w = __atomic_load_n(&a.nworkers[rr], __ATOMIC_RELAXED);
+      if (w < bw) {
+        bw = w;
+        best = rr;
+        if (w <= 1) break;
+      }
+    }
+    if (best < 0) return -1;
+    if (atomicAdd(&a.nworkers[best], 1) < a.workers) return best;
+    atomicSub(&a.nworkers[best], 1);
+  }
+  return -1;
+}
+
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
   __shared__ Ply ply[MAXPLY + 2];
   __shared__ u32 mmove[MSTACK];
-  __shared__ s32 mscore[MSTACK];
+  __shared__ int16_t mscore[MSTACK];
   __shared__ u8 mflag[MSTACK];
   __shared__ u32 killers[MAXPLY + 2][2];
-  __shared__ int sh_root;
 
   const int lane = threadIdx.x;
   const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
 
   for (;;) {
-    // ---------------------------------------------------------------- pick a root
-    if (lane == 0) {
-      int r = atomicAdd(a.queue, 1);
-      if (r < a.nroots) {
-        atomicAdd(&a.nworkers[r], 1);
-      } else {
-        r = -1;
-        for (int k = 0; k < a.nroots && a.workers > 1; k++) {
-          const int rr = (hstart + k) % a.nroots;
-          if (__atomic_load_n(&a.done[rr], __ATOMIC_RELAXED)) continue;
-          if (__atomic_load_n(&a.nworkers[rr], __ATOMIC_RELAXED) >= a.workers) continue;
-          if (atomicAdd(&a.nworkers[rr], 1) < a.workers) {
-            r = rr;
-            break;
-          }
-          atomicSub(&a.nworkers[rr], 1);
-        }
-      }
-      if (*a.abort_flag) r = -1;
-      sh_root = r;
-    }
-    wsync();
-    const int r = sh_root;
-    wsync();
+    int rr = -1;
+    if (lane == 0) rr = *a.abort_flag ? -1 : pick_root(a, hstart);
+    const int r = uf(rr);
     if (r < 0) return;
 
     // ---------------------------------------------------------------- root set-up
     const u64 *q = a.roots + (size_t)r * 8;
     Pos root;
-    root.b0 = q[0]; root.b1 = q[1]; root.b2 = q[2]; root.b3 = q[3]; root.cr = q[4];
+    root.b0 = q[0];
+    root.b1 = q[1];
+    root.b2 = q[2];
+    root.b3 = q[3];
+    root.cr = q[4];
     root.stm = (u32)(q[7] & 1u);
     const u64 roothash = q[5];
     const u32 roothmc = (u32)(q[6] & 0xFF);
     for (int k = lane; k < (MAXPLY + 2) * 2; k += 64) killers[k >> 1][k & 1] = 0;
-    if (lane == 0) {
-      ply[0].qs = 0;
-      ply[0].mbeg = 0;
-      ply[0].mcnt = 0;
-    }
     wsync();
 
     u64 nodes = 0;
@@ -273,7 +315,6 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     u32 final_move = 0;
 
     for (int id = 1; id <= a.depth && !aborted && !failed; id++) {
-      // root node
       Node n;
       n.p = root;
       n.hash = roothash;
@@ -288,6 +329,11 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
       n.busy = false;
       n.move = 0;
       n.best = 0;
+      n.qs = false;
+      n.kic = false;
+      n.iter = 1;
+      n.ndefer = 0;
+      n.searched = 0;
       int sd = 1;
       bool parent_qs = false;
       s32 rootscore = 0;  // DRAWSCORE until a root move raises alpha (src/zeta.cl:986)
@@ -305,11 +351,17 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
               d = __atomic_load_n(&a.done[r], __ATOMIC_RELAXED) | __atomic_load_n(a.abort_flag, __ATOMIC_RELAXED);
               if (a.max_nodes && nodes > a.max_nodes) d = 1;
             }
-            if (__shfl(d, 0, 64)) {
+            if (uf(d)) {
               aborted = true;
               break;
             }
           }
+          const int plyn = sd - 1;
+          // issue the TT load early: its latency overlaps the analysis below
+          const bool early = a.ttmask && sd > 1 && n.depth >= 1;
+          const u64 key = tt_key(n.hash, n.p.cr);
+          u32x4 tte = {0u, 0u, 0u, 0u};
+          if (early) tte = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
           Legal L;
           analyse(n.p, L);
           n.kic = L.checkers != 0;
@@ -319,7 +371,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
           n.iter = 1;
           n.ndefer = 0;
           n.searched = 0;
-          // move generation, lane = square; wave prefix sum compacts the lists
+          // move generation, lane = square; a wave prefix sum compacts the lists
           u64 t = 0;
           if ((L.own >> lane) & 1) t = piece_targets(n.p, L, lane, n.qs);
           int total;
@@ -332,23 +384,24 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
           }
           {
             int k = n.mbeg + off;
+            const u32 k0 = killers[sd][0], k1 = killers[sd][1];
             while (t) {
               const int to = lsb(t);
               t &= t - 1;
               const u32 m = make_code(n.p, lane, to);
               s32 sc = order_score(n.p.stm, m);
+              sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
               if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
-                if (m == killers[sd][0]) sc = KILLER0_SCORE;
-                else if (m == killers[sd][1]) sc = KILLER1_SCORE;
+                if (m == k0) sc = KILLER0_SCORE;
+                else if (m == k1) sc = KILLER1_SCORE;
               }
               mmove[k] = m;
-              mscore[k] = sc;
+              mscore[k] = (int16_t)sc;
               mflag[k] = 0;
               k++;
             }
           }
           // terminal / draw tests in the reference's order (src/zeta.cl:1554-1609)
-          const int plyn = sd - 1;
           bool draw = false;
           if (total > 0 && !n.qs && sd > 1) {
             // in-path repetition against plies plyn-2, plyn-4, .. >= plyn-hmc
@@ -384,9 +437,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             // TT probe: move for ordering; score under the exact-depth rule
             u32 ttmove = 0;
             if (a.ttmask && !n.qs && sd > 1) {
-              const u64 key = tt_key(n.hash, n.p.cr);
-              const u32x4 e = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
-              const u64 w0 = ((u64)e.y << 32) | e.x, w1 = ((u64)e.w << 32) | e.z;
+              if (!early) tte = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
+              const u64 w0 = uf(((u64)tte.y << 32) | tte.x), w1 = uf(((u64)tte.w << 32) | tte.z);
               c_probe++;
               if ((w0 ^ w1) == key && ((w1 >> 57) & 1)) {
                 c_hit++;
@@ -407,7 +459,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             }
             if (state == S_NEXT && ttmove) {
               for (int k = lane; k < total; k += 64)
-                if (mmove[n.mbeg + k] == ttmove) mscore[n.mbeg + k] = TTMOVE_SCORE;
+                if (mmove[n.mbeg + k] == ttmove) mscore[n.mbeg + k] = (int16_t)TTMOVE_SCORE;
             }
           }
           wsync();
@@ -415,17 +467,18 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         }
 
         if (state == S_NEXT) {
-          // ---------------- NEXT: pick the best remaining move
-          u64 bestkey = 0;
+          // ---------------- NEXT: pick the best remaining move (wave arg-max)
+          u32 bestkey = 0;
           for (int k = lane; k < n.mcnt; k += 64) {
             const s32 sc = mscore[n.mbeg + k];
-            const bool ok = sc != USED && (n.iter == 1 ? mflag[n.mbeg + k] == 0 : mflag[n.mbeg + k] != 0);
+            const u8 fl = mflag[n.mbeg + k];
+            const bool ok = sc != USED && (n.iter == 1 ? fl == 0 : fl != 0);
             if (ok) {
-              const u64 key = ((u64)((u32)sc ^ 0x80000000u) << 32) | (u64)(0xFFFFu - (u32)k) | (1ULL << 16);
+              const u32 key = ((u32)(sc + 32768) << 16) | (0xFFFFu - (u32)k);
               bestkey = key > bestkey ? key : bestkey;
             }
           }
-          bestkey = wave_max_u64(bestkey);
+          bestkey = wave_max_u32(bestkey);
           if (bestkey == 0) {
             if (n.iter == 1 && n.ndefer > 0) {
               n.iter = 2;
@@ -436,11 +489,11 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             state = S_RETURN;
             continue;
           }
-          const int k = n.mbeg + (int)(0xFFFFu - (u32)(bestkey & 0xFFFF));
-          const u32 m = mmove[k];
-          const s32 sk = mscore[k];
+          const int k = n.mbeg + (int)(0xFFFFu - (bestkey & 0xFFFFu));
+          const u32 m = uf(mmove[k]);
+          const s32 sk = (s32)(bestkey >> 16) - 32768;
           wsync();
-          if (lane == 0) mscore[k] = USED;
+          if (lane == 0) mscore[k] = (int16_t)USED;
           const Pos c = apply(n.p, m, false);
           u64 ch, cph;
           child_hash(n, m, c, ch, cph);
@@ -450,10 +503,10 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             const u32 bidx = (u32)(tt_key(ch, c.cr) >> 40) & a.busymask;
             u32 b = 0;
             if (lane == 0) b = __atomic_load_n(&a.busy[bidx], __ATOMIC_RELAXED);
-            b = __shfl(b, 0, 64);
+            b = uf(b);
             if (n.iter == 1 && n.searched > 0 && b) {
               if (lane == 0) {
-                mscore[k] = sk;
+                mscore[k] = (int16_t)sk;
                 mflag[k] = 1;
               }
               n.ndefer++;
@@ -497,14 +550,14 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         }
 
         // ---------------- RETURN v from node sd
-        if (n.busy) {
-          const u32 bidx = (u32)(tt_key(n.hash, n.p.cr) >> 40) & a.busymask;
-          if (lane == 0) atomicSub(&a.busy[bidx], 1u);
-        }
+        if (n.busy) busy_release(a, n.hash, n.p.cr, lane);
         if (sd == 1) break;
         sd--;
         wsync();
-        load(ply[sd], n);
+        {
+          const Pos child = n.p;
+          restore(ply[sd], child, n);
+        }
         const s32 sc = -v;
         if (sc > n.alpha) {
           n.alpha = sc;
@@ -530,17 +583,10 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
 
       if (aborted || failed) {
         // release the ABDADA marks still held along the path
-        if (n.busy) {
-          const u32 bidx = (u32)(tt_key(n.hash, n.p.cr) >> 40) & a.busymask;
-          if (lane == 0) atomicSub(&a.busy[bidx], 1u);
-        }
+        if (n.busy) busy_release(a, n.hash, n.p.cr, lane);
         wsync();
-        for (int k = 2; k < sd; k++) {
-          if (ply[k].busy) {
-            const u32 bidx = (u32)(tt_key(ply[k].hash, ply[k].cr) >> 40) & a.busymask;
-            if (lane == 0) atomicSub(&a.busy[bidx], 1u);
-          }
-        }
+        for (int k = 2; k < sd; k++)
+          if (ply[k].busy) busy_release(a, uf(ply[k].hash), uf(ply[k].cr), lane);
         break;
       }
       final_score = (rootscore == INF || rootscore == -INF) ? 0 : rootscore;  // src/search.c:137
