@@ -115,6 +115,29 @@ int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, 
  * Each probe / store moves one 16-byte entry. */
 int zeta_search_counters(uint64_t *out, int n);
 
+/* ------------------------------------------------ learned move ordering
+ * No counterpart in the reference (its ordering is fixed scores,
+ * src/zeta.cl:2059-2104).  A 908 -> 256 -> 32 -> 1 ReLU network with BF16
+ * weights and FP32 accumulation scores every legal move at interior nodes
+ * when zeta_search_opts.use_mlp is set.  Parameters cross the boundary as
+ * FP32 in this order: W1[908][256], b1[256], W2[256][32], b2[32], W3[32], b3.
+ * Training pairs (48 B each: 4 x u64 position planes, u32 side to move,
+ * u32 cutoff move, u32 failed move, u32 depth) are logged by searches with
+ * log_records set, or supplied with zeta_records_put. */
+#define ZETA_MLP_NPARAMS 240961
+int zeta_mlp_init(uint64_t seed);
+int zeta_mlp_set(const float *params, size_t n);
+int zeta_mlp_get(float *params, size_t n);
+/* loss and gradient (ABI order) of npairs pairs (multiple of 16), no update */
+int zeta_mlp_grad(const void *pairs, int npairs, float *loss, float *grads, size_t n);
+/* Adam steps on random batches of the logged pairs; losses[steps] may be NULL */
+int zeta_mlp_train(int steps, int pairs_per_step, float lr, uint64_t seed, float *losses);
+int zeta_records_reserve(uint64_t capacity);
+int zeta_records_count(uint64_t *n);
+int zeta_records_clear(void);
+int zeta_records_get(void *out, uint64_t max, uint64_t *n);
+int zeta_records_put(const void *in, uint64_t n);
+
 /* ------------------------------------------------------------ device memory
  * Device buffers for callers that keep inputs resident on the GPU (the
  * *_device entry points).  Plain pointers and sizes; all copies are

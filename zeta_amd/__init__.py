@@ -75,6 +75,15 @@ def lib():
         L.zeta_set_timing.argtypes = [i32]
         L.zeta_search_batch.argtypes = [vp, i32, ctypes.POINTER(SearchOpts), vp]
         L.zeta_search_counters.argtypes = [vp, i32]
+        L.zeta_mlp_init.argtypes = [u64]
+        L.zeta_mlp_set.argtypes = [vp, ctypes.c_size_t]
+        L.zeta_mlp_get.argtypes = [vp, ctypes.c_size_t]
+        L.zeta_mlp_grad.argtypes = [vp, i32, vp, vp, ctypes.c_size_t]
+        L.zeta_mlp_train.argtypes = [i32, i32, ctypes.c_float, u64, vp]
+        L.zeta_records_reserve.argtypes = [u64]
+        L.zeta_records_count.argtypes = [vp]
+        L.zeta_records_get.argtypes = [vp, u64, vp]
+        L.zeta_records_put.argtypes = [vp, u64]
         L.zeta_dev_alloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         L.zeta_dev_free.argtypes = [vp]
         L.zeta_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
@@ -83,7 +92,9 @@ def lib():
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
                   "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
                   "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize",
-                  "zeta_search_batch", "zeta_search_counters"):
+                  "zeta_search_batch", "zeta_search_counters", "zeta_mlp_init", "zeta_mlp_set",
+                  "zeta_mlp_get", "zeta_mlp_grad", "zeta_mlp_train", "zeta_records_reserve",
+                  "zeta_records_count", "zeta_records_clear", "zeta_records_get", "zeta_records_put"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -232,3 +243,75 @@ def stats():
 def stats_reset():
     ensure_init()
     _check(lib().zeta_stats_reset(), "zeta_stats_reset")
+
+
+# ------------------------------------------------------------ learned ordering
+MLP_NPARAMS = 240961
+PAIR_DTYPE = np.dtype([("b", "<u8", (4,)), ("stm", "<u4"), ("mpos", "<u4"), ("mneg", "<u4"), ("depth", "<u4")])
+
+
+def mlp_init(seed=1):
+    ensure_init()
+    _check(lib().zeta_mlp_init(int(seed)), "zeta_mlp_init")
+
+
+def mlp_set(params):
+    ensure_init()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    _check(lib().zeta_mlp_set(_ptr(p), p.size), "zeta_mlp_set")
+
+
+def mlp_get():
+    ensure_init()
+    p = np.zeros(MLP_NPARAMS, dtype=np.float32)
+    _check(lib().zeta_mlp_get(_ptr(p), p.size), "zeta_mlp_get")
+    return p
+
+
+def mlp_grad(pairs):
+    ensure_init()
+    pairs = np.ascontiguousarray(pairs, dtype=PAIR_DTYPE)
+    g = np.zeros(MLP_NPARAMS, dtype=np.float32)
+    loss = ctypes.c_float(0)
+    _check(lib().zeta_mlp_grad(_ptr(pairs), len(pairs), ctypes.byref(loss), _ptr(g), g.size), "zeta_mlp_grad")
+    return float(loss.value), g
+
+
+def mlp_train(steps, pairs_per_step=2048, lr=1e-3, seed=1):
+    ensure_init()
+    losses = np.zeros(max(steps, 1), dtype=np.float32)
+    _check(lib().zeta_mlp_train(int(steps), int(pairs_per_step), float(lr), int(seed), _ptr(losses)),
+           "zeta_mlp_train")
+    return losses[:steps]
+
+
+def records_reserve(capacity):
+    ensure_init()
+    _check(lib().zeta_records_reserve(int(capacity)), "zeta_records_reserve")
+
+
+def records_count():
+    ensure_init()
+    n = ctypes.c_uint64(0)
+    _check(lib().zeta_records_count(ctypes.byref(n)), "zeta_records_count")
+    return int(n.value)
+
+
+def records_clear():
+    ensure_init()
+    _check(lib().zeta_records_clear(), "zeta_records_clear")
+
+
+def records_get(maxn=None):
+    ensure_init()
+    n = records_count() if maxn is None else int(maxn)
+    out = np.zeros(n, dtype=PAIR_DTYPE)
+    got = ctypes.c_uint64(0)
+    _check(lib().zeta_records_get(_ptr(out), n, ctypes.byref(got)), "zeta_records_get")
+    return out[:got.value]
+
+
+def records_put(pairs):
+    ensure_init()
+    pairs = np.ascontiguousarray(pairs, dtype=PAIR_DTYPE)
+    _check(lib().zeta_records_put(_ptr(pairs), len(pairs)), "zeta_records_put")

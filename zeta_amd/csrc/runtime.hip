@@ -147,6 +147,7 @@ extern "C" int zeta_init(const zeta_config *cfg) {
 // release hooks of the subsystems (defined in their own files)
 void perft_release();
 void search_release();
+void mlp_release();
 
 extern "C" int zeta_shutdown(void) {
   Ctx &c = ctx();
@@ -155,6 +156,7 @@ extern "C" int zeta_shutdown(void) {
   time_collect();
   perft_release();
   search_release();
+  mlp_release();
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
   c.event_pool.clear();
   (void)hipStreamDestroy(c.stream);
