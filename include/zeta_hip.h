@@ -111,7 +111,8 @@ typedef struct {
 
 int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, zeta_result *res);
 /* Counters of the last zeta_search_batch: [0] TT probes, [1] TT key hits,
- * [2] TT scores used, [3] TT stores, [4] ABDADA deferrals, [5] MAXPLY cuts.
+ * [2] TT scores used, [3] TT stores, [4] ABDADA deferrals, [5] MAXPLY cuts,
+ * [6] nodes ordered by the network, [7] training pairs logged.
  * Each probe / store moves one 16-byte entry. */
 int zeta_search_counters(uint64_t *out, int n);
 
@@ -132,6 +133,9 @@ int zeta_mlp_get(float *params, size_t n);
 int zeta_mlp_grad(const void *pairs, int npairs, float *loss, float *grads, size_t n);
 /* Adam steps on random batches of the logged pairs; losses[steps] may be NULL */
 int zeta_mlp_train(int steps, int pairs_per_step, float lr, uint64_t seed, float *losses);
+/* network scores (as the search uses them: 1/256 resolution, clamped) of
+ * moves[offsets[i] .. offsets[i+1]) from position i */
+int zeta_mlp_score(const zeta_pos *pos, int npos, const uint32_t *moves, const int32_t *offsets, float *scores);
 int zeta_records_reserve(uint64_t capacity);
 int zeta_records_count(uint64_t *n);
 int zeta_records_clear(void);

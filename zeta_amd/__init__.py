@@ -81,6 +81,7 @@ def lib():
         L.zeta_mlp_grad.argtypes = [vp, i32, vp, vp, ctypes.c_size_t]
         L.zeta_mlp_train.argtypes = [i32, i32, ctypes.c_float, u64, vp]
         L.zeta_records_reserve.argtypes = [u64]
+        L.zeta_mlp_score.argtypes = [vp, i32, vp, vp, vp]
         L.zeta_records_count.argtypes = [vp]
         L.zeta_records_get.argtypes = [vp, u64, vp]
         L.zeta_records_put.argtypes = [vp, u64]
@@ -94,7 +95,8 @@ def lib():
                   "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize",
                   "zeta_search_batch", "zeta_search_counters", "zeta_mlp_init", "zeta_mlp_set",
                   "zeta_mlp_get", "zeta_mlp_grad", "zeta_mlp_train", "zeta_records_reserve",
-                  "zeta_records_count", "zeta_records_clear", "zeta_records_get", "zeta_records_put"):
+                  "zeta_records_count", "zeta_records_clear", "zeta_records_get", "zeta_records_put",
+                  "zeta_mlp_score"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -184,7 +186,8 @@ def search_counters():
     ensure_init()
     out = np.zeros(8, dtype=np.uint64)
     _check(lib().zeta_search_counters(_ptr(out), 8), "zeta_search_counters")
-    keys = ("tt_probes", "tt_hits", "tt_cuts", "tt_stores", "abdada_defers", "maxply_cuts")
+    keys = ("tt_probes", "tt_hits", "tt_cuts", "tt_stores", "abdada_defers", "maxply_cuts", "mlp_nodes",
+            "records_logged")
     return {k: int(out[i]) for i, k in enumerate(keys)}
 
 
@@ -315,3 +318,15 @@ def records_put(pairs):
     ensure_init()
     pairs = np.ascontiguousarray(pairs, dtype=PAIR_DTYPE)
     _check(lib().zeta_records_put(_ptr(pairs), len(pairs)), "zeta_records_put")
+
+
+def mlp_score(pos, moves_per_pos):
+    """Scores the search's network assigns to the given moves of each position."""
+    ensure_init()
+    pos = as_positions(pos)
+    offs = np.zeros(len(pos) + 1, dtype=np.int32)
+    offs[1:] = np.cumsum([len(m) for m in moves_per_pos])
+    mv = np.ascontiguousarray(np.concatenate([np.asarray(m, dtype=np.uint32) for m in moves_per_pos]))
+    out = np.zeros(len(mv), dtype=np.float32)
+    _check(lib().zeta_mlp_score(_ptr(pos), len(pos), _ptr(mv), _ptr(offs), _ptr(out)), "zeta_mlp_score")
+    return [out[offs[i]:offs[i + 1]] for i in range(len(pos))]

@@ -33,6 +33,7 @@ constexpr int P_B3 = P_W3 + NH2;          // 1
 
 // device weight set used by the search and by training
 struct Weights {
+  const u16 *w1f;   // BF16 [928][256]: W1, unit-contiguous (row gathers in the search)
   const u16 *w1t;   // BF16 [256][928]: W1 transposed, input-contiguous
   const u16 *w2t;   // BF16 [32][256]
   const float *b1;  // [256]

@@ -203,9 +203,14 @@ __global__ void k_adam(float *prm, const float *__restrict__ grad, float *m, flo
 }
 
 // BF16 working copies of the master weights
-__global__ void k_refresh(const float *__restrict__ prm, u16 *w1t, u16 *w2t, u16 *w2) {
+__global__ void k_refresh(const float *__restrict__ prm, u16 *w1t, u16 *w1f, u16 *w2t, u16 *w2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < NH1 * NINP) w1t[i] = f2bf(prm[I_W1T + i]);
+  if (i < NH1 * NINP) {
+    const u16 b = f2bf(prm[I_W1T + i]);
+    w1t[i] = b;
+    const int u = i / NINP, f = i % NINP;
+    w1f[f * NH1 + u] = b;
+  }
   if (i < NH2 * NH1) {
     const u16 b = f2bf(prm[I_W2T + i]);
     w2t[i] = b;
@@ -244,7 +249,7 @@ __global__ void k_sample(u32 *idx, int P, u64 nrec, u64 seed) {
 }
 
 struct MlpState {
-  zr::DevBuf prm, grad, m, v, w1t, w2t, w2;          // weights
+  zr::DevBuf prm, grad, m, v, w1t, w1f, w2t, w2;     // weights
   zr::DevBuf X, Xt, Z1, H1, H1t, Z2, dZ2, dZ2t, dH1, dZ1t, idx, loss, recs_scratch;
   int cap_pairs = 0;
   bool ready = false;
@@ -256,7 +261,8 @@ int ensure_weights() {
   int r;
   if ((r = zr::ensure(g_ms.prm, NINT * 4)) || (r = zr::ensure(g_ms.grad, NINT * 4)) ||
       (r = zr::ensure(g_ms.m, NINT * 4)) || (r = zr::ensure(g_ms.v, NINT * 4)) ||
-      (r = zr::ensure(g_ms.w1t, NH1 * NINP * 2)) || (r = zr::ensure(g_ms.w2t, NH2 * NH1 * 2)) ||
+      (r = zr::ensure(g_ms.w1t, NH1 * NINP * 2)) || (r = zr::ensure(g_ms.w1f, NH1 * NINP * 2)) ||
+      (r = zr::ensure(g_ms.w2t, NH2 * NH1 * 2)) ||
       (r = zr::ensure(g_ms.w2, NH2 * NH1 * 2)))
     return r;
   return ZETA_OK;
@@ -325,7 +331,7 @@ int fwd_bwd(const PairRec *recs, int P) {
 
 int refresh() {
   ZLAUNCH(zr::K_MLP_ADAM, NH1 * NINP, k_refresh, nb((size_t)NH1 * NINP, 256), 256, 0, (const float *)g_ms.prm.p,
-          (u16 *)g_ms.w1t.p, (u16 *)g_ms.w2t.p, (u16 *)g_ms.w2.p);
+          (u16 *)g_ms.w1t.p, (u16 *)g_ms.w1f.p, (u16 *)g_ms.w2t.p, (u16 *)g_ms.w2.p);
   return ZETA_OK;
 }
 
@@ -383,11 +389,18 @@ int ensure(u64 cap) {
 }
 }  // namespace zrec
 
+void *records_buffer(u64 *cap, u64 **count) {
+  *cap = zrec::g_cap;
+  *count = (u64 *)zrec::g_count.p;
+  return zrec::g_recs.p;
+}
+
 const zm::Weights *mlp_weights() {
   static zm::Weights w;
   if (!g_ms.ready) return nullptr;
   const float *prm = (const float *)g_ms.prm.p;
   w.w1t = (const u16 *)g_ms.w1t.p;
+  w.w1f = (const u16 *)g_ms.w1f.p;
   w.w2t = (const u16 *)g_ms.w2t.p;
   w.b1 = prm + I_B1;
   w.b2 = prm + I_B2;
@@ -397,7 +410,7 @@ const zm::Weights *mlp_weights() {
 }
 
 void mlp_release() {
-  zr::DevBuf *bufs[] = {&g_ms.prm, &g_ms.grad, &g_ms.m,   &g_ms.v,    &g_ms.w1t,  &g_ms.w2t, &g_ms.w2,
+  zr::DevBuf *bufs[] = {&g_ms.prm, &g_ms.grad, &g_ms.m,   &g_ms.v,    &g_ms.w1t,  &g_ms.w1f, &g_ms.w2t, &g_ms.w2,
                         &g_ms.X,   &g_ms.Xt,   &g_ms.Z1,  &g_ms.H1,   &g_ms.H1t,  &g_ms.Z2,  &g_ms.dZ2,
                         &g_ms.dZ2t, &g_ms.dH1, &g_ms.dZ1t, &g_ms.idx, &g_ms.loss, &g_ms.recs_scratch};
   for (auto *b : bufs) zr::release(*b);

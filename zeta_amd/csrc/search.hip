@@ -34,6 +34,7 @@
 
 #include "chess.h"
 #include "evaltab.h"
+#include "mlp.h"
 #include "runtime.h"
 
 using namespace zc;
@@ -73,10 +74,21 @@ struct SearchArgs {
   const u32 *abort_flag;
   u64 max_nodes;  // per worker and root (0 = none)
   u64 *ctr;       // batch counters, see zeta_search_counters
+  // learned ordering
+  int use_mlp;
+  int mlp_min_depth;
+  zm::Weights w;
+  // training-pair log (cut nodes): PairRec[cap], counter
+  u64 *recs;
+  u64 *rec_count;
+  u64 rec_cap;
 };
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
 // batch counters (device, summed per worker at the end of each root)
-enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_NCTR = 8 };
+enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_MLP_NODES, C_RECORDS, C_NCTR = 8 };
 
 // One ply of a worker's stack in LDS (64 B).  The board itself is not kept:
 // a parent board is recovered with undo() when a child returns.
@@ -277,12 +289,136 @@ __device__ int pick_root(const SearchArgs &a, int hstart) {
   return -1;
 }
 
+__device__ __forceinline__ void add_row(float x[8], const u16 *row) {
+  const uint4 q = *(const uint4 *)row;  // 8 BF16
+  const u32 v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    x[2 * j] += zm::bf2f((u16)(v[j] & 0xFFFF));
+    x[2 * j + 1] += zm::bf2f((u16)(v[j] >> 16));
+  }
+}
+
+// Learned ordering: network score of every move of the node (moves
+// [mbeg, mbeg+total) in LDS) written into mscore.  Layer 1's position part is
+// accumulated once (lane = 4 hidden units); each chunk of 32 moves then runs
+// layer 2 as 16 MFMA steps whose A operand (move x 8 hidden units, BF16) is
+// built in registers from the position part plus the move's 3-4 feature rows.
+__device__ void mlp_score_moves(const SearchArgs &a, const Pos &p, int mbeg, int total, int lane, float *hpos,
+                                const u32 *mmove, int16_t *mscore) {
+  const zm::Weights &w = a.w;
+  float h4[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) h4[j] = w.b1[lane * 4 + j];
+  u64 occ = p.b1 | p.b2 | p.b3;
+  while (occ) {
+    const int sq = lsb(occ);
+    occ &= occ - 1;
+    const int f = zm::feat_ps(piece_at(p, sq), sq, p.stm);
+    const uint2 q = *(const uint2 *)(w.w1f + (size_t)f * zm::NH1 + lane * 4);
+    h4[0] += zm::bf2f((u16)(q.x & 0xFFFF));
+    h4[1] += zm::bf2f((u16)(q.x >> 16));
+    h4[2] += zm::bf2f((u16)(q.y & 0xFFFF));
+    h4[3] += zm::bf2f((u16)(q.y >> 16));
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) hpos[lane * 4 + j] = h4[j];
+  wsync();
+  const int col = lane & 31, half = lane >> 5;
+  const float b2o = w.b2[col], w3o = w.w3[col], b3 = w.b3[0];
+  for (int c0 = 0; c0 < total; c0 += 32) {
+    const int mi = c0 + col;
+    const u32 m = mi < total ? mmove[mbeg + mi] : 0u;
+    const int ff = m ? zm::feat_from(m, p.stm) : 0, ft = m ? zm::feat_to(m, p.stm) : 0;
+    const int fp = m ? zm::feat_piece(m) : 0, fc = m ? zm::feat_capt(m) : -1;
+    f32x16 acc = {};
+    for (int ks = 0; ks < zm::NH1 / 16; ks++) {
+      const int kb = ks * 16 + half * 8;
+      float x[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) x[j] = hpos[kb + j];
+      if (m) {
+        add_row(x, w.w1f + (size_t)ff * zm::NH1 + kb);
+        add_row(x, w.w1f + (size_t)ft * zm::NH1 + kb);
+        add_row(x, w.w1f + (size_t)fp * zm::NH1 + kb);
+        if (fc >= 0) add_row(x, w.w1f + (size_t)fc * zm::NH1 + kb);
+      }
+      bf16x8 av;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const u16 hb = zm::f2bf(x[j] > 0.f ? x[j] : 0.f);
+        av[j] = __builtin_bit_cast(__bf16, hb);
+      }
+      const bf16x8 bv = *(const bf16x8 *)(w.w2t + (size_t)col * zm::NH1 + kb);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    // layer 3: register i of lane l holds z2[move 8*(i/4)+4*half+i%4][unit col]
+    float t[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const float z = acc[i] + b2o;
+      t[i] = z > 0.f ? w3o * z : 0.f;
+    }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1)
+#pragma unroll
+      for (int i = 0; i < 16; i++) t[i] += __shfl_xor(t[i], o, 64);
+    if (col == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = 8 * (i >> 2) + 4 * half + (i & 3);
+        if (c0 + row < total) {
+          float sc = (t[i] + b3) * 256.f;
+          sc = sc > 31000.f ? 31000.f : (sc < -31000.f ? -31000.f : sc);
+          mscore[mbeg + c0 + row] = (int16_t)__float2int_rn(sc);
+        }
+      }
+    }
+  }
+  wsync();
+}
+
+// training pairs at a cut node: the cutoff move against every move searched
+// before it at this node (scores marked USED), wave-compacted
+__device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u32 *mmove, const int16_t *mscore,
+                          u32 &nrec) {
+  for (int c0 = 0; c0 < n.mcnt; c0 += 64) {
+    const int k = c0 + lane;
+    bool neg = false;
+    u32 m = 0;
+    if (k < n.mcnt) {
+      m = mmove[n.mbeg + k];
+      neg = mscore[n.mbeg + k] == (int16_t)USED && m != n.move;
+    }
+    const u64 mask = __ballot(neg);
+    if (!mask) continue;
+    const u32 cnt = (u32)popc(mask);
+    u64 base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long *)a.rec_count, (unsigned long long)cnt);
+    base = uf(base);
+    if (neg) {
+      const u64 slot = base + (u64)popc(mask & ((1ULL << lane) - 1));
+      if (slot < a.rec_cap) {
+        u64 *rp = a.recs + slot * 6;
+        rp[0] = n.p.b0;
+        rp[1] = n.p.b1;
+        rp[2] = n.p.b2;
+        rp[3] = n.p.b3;
+        rp[4] = (u64)n.p.stm | ((u64)n.move << 32);
+        rp[5] = (u64)m | ((u64)(u32)n.depth << 32);
+      }
+    }
+    nrec += cnt;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
   __shared__ Ply ply[MAXPLY + 2];
   __shared__ u32 mmove[MSTACK];
   __shared__ int16_t mscore[MSTACK];
   __shared__ u8 mflag[MSTACK];
   __shared__ u32 killers[MAXPLY + 2][2];
+  __shared__ float hpos[zm::NH1];
 
   const int lane = threadIdx.x;
   const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
@@ -308,7 +444,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     wsync();
 
     u64 nodes = 0;
-    u32 c_probe = 0, c_hit = 0, c_cut = 0, c_store = 0, c_defer = 0, c_maxply = 0;
+    u32 c_probe = 0, c_hit = 0, c_cut = 0, c_store = 0, c_defer = 0, c_maxply = 0, c_mlp = 0, c_rec = 0;
     int poll = 0;
     bool aborted = false, failed = false;
     s32 final_score = 0;
@@ -400,6 +536,11 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
               mflag[k] = 0;
               k++;
             }
+          }
+          if (a.use_mlp && !n.qs && n.depth >= a.mlp_min_depth && total > 1) {
+            wsync();
+            mlp_score_moves(a, n.p, n.mbeg, total, lane, hpos, mmove, mscore);
+            c_mlp++;
           }
           // terminal / draw tests in the reference's order (src/zeta.cl:1554-1609)
           bool draw = false;
@@ -568,6 +709,7 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
           }
         }
         if (n.alpha >= n.beta) {
+          if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 2) log_pairs(a, n, lane, mmove, mscore, c_rec);
           if (((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0 && killers[sd][0] != n.move) {
             killers[sd][1] = killers[sd][0];
             killers[sd][0] = n.move;
@@ -603,6 +745,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
       atomicAdd((unsigned long long *)&a.ctr[C_TT_STORE], (unsigned long long)c_store);
       atomicAdd((unsigned long long *)&a.ctr[C_DEFER], (unsigned long long)c_defer);
       atomicAdd((unsigned long long *)&a.ctr[C_MAXPLY], (unsigned long long)c_maxply);
+      atomicAdd((unsigned long long *)&a.ctr[C_MLP_NODES], (unsigned long long)c_mlp);
+      atomicAdd((unsigned long long *)&a.ctr[C_RECORDS], (unsigned long long)c_rec);
       if (!aborted && !failed && atomicCAS(&a.done[r], 0u, 1u) == 0u) {
         a.score[r] = final_score;
         a.bestmove[r] = final_move;
@@ -612,6 +756,29 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     }
     wsync();
   }
+}
+
+// the search's learned-ordering routine applied to given moves (for tests and
+// external callers): one wave per position, up to MSTACK moves each
+__global__ __launch_bounds__(64) void k_mlp_score(SearchArgs a, const u64 *pos, const u32 *moves, const int *off,
+                                                 float *out) {
+  __shared__ u32 mmove[MSTACK];
+  __shared__ int16_t mscore[MSTACK];
+  __shared__ float hpos[zm::NH1];
+  const int lane = threadIdx.x, i = blockIdx.x;
+  const u64 *q = pos + (size_t)i * 8;
+  Pos p;
+  p.b0 = q[0];
+  p.b1 = q[1];
+  p.b2 = q[2];
+  p.b3 = q[3];
+  p.cr = q[4];
+  p.stm = (u32)(q[7] & 1u);
+  const int b = off[i], n = off[i + 1] - off[i];
+  for (int k = lane; k < n; k += 64) mmove[k] = moves[b + k];
+  wsync();
+  mlp_score_moves(a, p, 0, n, lane, hpos, mmove, mscore);
+  for (int k = lane; k < n; k += 64) out[b + k] = (float)mscore[k] / 256.f;
 }
 
 __global__ void k_clear_u64(u64 *p, size_t n) {
@@ -628,6 +795,9 @@ SearchState g_ss;
 inline unsigned nblocks(size_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
+
+const zm::Weights *mlp_weights();
+void *records_buffer(u64 *cap, u64 **count);
 
 void search_release() {
   zr::release(g_ss.tt); zr::release(g_ss.busy); zr::release(g_ss.roots); zr::release(g_ss.queue);
@@ -703,6 +873,26 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.abort_flag = (const u32 *)g_ss.abortf.p;
   sa.max_nodes = opts->max_nodes;
   sa.ctr = (u64 *)g_ss.ctr.p;
+  sa.use_mlp = 0;
+  sa.mlp_min_depth = opts->mlp_min_depth;
+  memset(&sa.w, 0, sizeof(sa.w));
+  if (opts->use_mlp) {
+    const zm::Weights *w = mlp_weights();
+    if (!w) return zr::fail(ZETA_ESTATE, "search: use_mlp set but no network loaded (zeta_mlp_init/zeta_mlp_set)");
+    sa.w = *w;
+    sa.use_mlp = 1;
+  }
+  sa.recs = nullptr;
+  sa.rec_count = nullptr;
+  sa.rec_cap = 0;
+  if (opts->log_records) {
+    u64 cap = 0, *cnt = nullptr;
+    void *recs = records_buffer(&cap, &cnt);
+    if (!recs || !cap) return zr::fail(ZETA_ESTATE, "search: log_records set but no record buffer (zeta_records_reserve)");
+    sa.recs = (u64 *)recs;
+    sa.rec_count = cnt;
+    sa.rec_cap = cap;
+  }
   ZLAUNCH(zr::K_SEARCH, (u64)n, k_search, grid, 64, 0, sa);
 
   std::vector<s32> score(n), dd(n);
@@ -736,5 +926,42 @@ extern "C" int zeta_search_counters(uint64_t *out, int n) {
   if (r) return r;
   if (!out || n < 0) return zr::fail(ZETA_EINVAL, "zeta_search_counters: bad arguments");
   for (int i = 0; i < n && i < C_NCTR; i++) out[i] = g_ss.last_ctr[i];
+  return ZETA_OK;
+}
+
+extern "C" int zeta_mlp_score(const zeta_pos *pos, int npos, const uint32_t *moves, const int32_t *offsets,
+                              float *scores) {
+  int r = zr::require_up();
+  if (r) return r;
+  const zm::Weights *w = mlp_weights();
+  if (!w) return zr::fail(ZETA_ESTATE, "zeta_mlp_score: no network loaded");
+  if (npos <= 0 || !pos || !moves || !offsets || !scores) return zr::fail(ZETA_EINVAL, "zeta_mlp_score: bad arguments");
+  for (int i = 0; i < npos; i++)
+    if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > MSTACK)
+      return zr::fail(ZETA_EINVAL, "zeta_mlp_score: bad offsets at %d", i);
+  const int nm = offsets[npos];
+  zr::DevBuf dp, dm, doff, dout;
+  if ((r = zr::ensure(dp, (size_t)npos * sizeof(zeta_pos))) || (r = zr::ensure(dm, (size_t)nm * 4 + 4)) ||
+      (r = zr::ensure(doff, (size_t)(npos + 1) * 4)) || (r = zr::ensure(dout, (size_t)nm * 4 + 4))) {
+    zr::release(dp); zr::release(dm); zr::release(doff); zr::release(dout);
+    return r;
+  }
+  hipStream_t st = zr::ctx().stream;
+  SearchArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.w = *w;
+  sa.use_mlp = 1;
+  hipError_t e = hipMemcpyAsync(dp.p, pos, (size_t)npos * sizeof(zeta_pos), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dm.p, moves, (size_t)nm * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(doff.p, offsets, (size_t)(npos + 1) * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mlp_score, dim3(npos), dim3(64), 0, st, sa, (const u64 *)dp.p, (const u32 *)dm.p,
+                       (const int *)doff.p, (float *)dout.p);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(scores, dout.p, (size_t)nm * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  zr::release(dp); zr::release(dm); zr::release(doff); zr::release(dout);
+  if (e != hipSuccess) return zr::hip_fail(e, "zeta_mlp_score", __FILE__, __LINE__);
   return ZETA_OK;
 }
