@@ -12,16 +12,23 @@ pytestmark = pytest.mark.gpu
 
 def _pairs(oracle, n, seed):
     rng = np.random.default_rng(seed)
-    pos = oracle.gen_playouts(200 + seed, n, 8, 40)
+    pos = oracle.gen_playouts(200 + seed, 2 * n, 8, 40)
     out = np.zeros(n, dtype=M.PAIR_DTYPE)
-    for i in range(n):
-        p = pos[i:i + 1]
+    i = 0
+    for j in range(len(pos)):
+        p = pos[j:j + 1]
         k = oracle.count_legal(p)
+        if k < 2:
+            continue
         a, b = rng.choice(k, 2, replace=False)
         out[i]["b"] = p["b"][0, :4]
         out[i]["stm"] = int(p["stm"][0])
         out[i]["mpos"] = oracle.nth_legal(p, int(a))
         out[i]["mneg"] = oracle.nth_legal(p, int(b))
+        i += 1
+        if i == n:
+            break
+    assert i == n
     return out
 
 

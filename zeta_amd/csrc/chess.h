@@ -241,6 +241,20 @@ ZD void analyse(const Pos &p, Legal &L) {
   L.cmask = n == 0 ? FULL : (n == 1 ? (checkers | chkray) : 0ULL);
 }
 
+// is the side to move in check?  (cheap: attackers of the king square only)
+ZD bool in_check(const Pos &p) {
+  const u64 occ = p.b1 | p.b2 | p.b3;
+  const u64 own = p.stm ? p.b0 : (p.b0 ^ occ);
+  const u64 opp = occ ^ own;
+  const u64 kbb = own & p.b1 & p.b2 & ~p.b3;
+  const u64 e = ~occ;
+  const u64 rq = p.b3 & (p.b1 ^ p.b2), bq = p.b3 & ~p.b1;
+  u64 att = (rook_att(kbb, e) & opp & rq) | (bishop_att(kbb, e) & opp & bq);
+  att |= knight_att(kbb) & opp & (~p.b1 & p.b2 & ~p.b3);
+  att |= pawn_att(kbb, p.stm) & opp & (p.b1 & ~p.b2 & ~p.b3);
+  return att != 0;
+}
+
 // is the side to move's king attacked after the e.p. capture from->to taking sqep?
 ZD bool ep_legal(const Legal &L, int from, int to, int sqep) {
   const u64 occ2 = (L.occ ^ bit(from) ^ bit(sqep)) | bit(to);

@@ -412,7 +412,7 @@ __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u3
   }
 }
 
-__global__ __launch_bounds__(64) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_search(SearchArgs a) {
   __shared__ Ply ply[MAXPLY + 2];
   __shared__ u32 mmove[MSTACK];
   __shared__ int16_t mscore[MSTACK];
@@ -498,15 +498,38 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
           const u64 key = tt_key(n.hash, n.p.cr);
           u32x4 tte = {0u, 0u, 0u, 0u};
           if (early) tte = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
-          Legal L;
-          analyse(n.p, L);
-          n.kic = L.checkers != 0;
+          n.kic = in_check(n.p);
           if (n.depth >= 0 && !parent_qs && (n.kic || is_promo(n.lastmove))) n.depth++;
           n.qs = n.depth <= 0;
           n.best = 0;
           n.iter = 1;
           n.ndefer = 0;
           n.searched = 0;
+          n.mcnt = 0;
+          if (n.qs && !n.kic) {
+            // quiescence stand pat before any move generation: the reference
+            // returns the static eval both on a stand-pat cutoff and when no
+            // capture exists (src/zeta.cl:1597-1605), so eval >= beta ends the
+            // node with the same value either way
+            const bool qdraw = sd > 1 && (n.hmc >= 100 || popc(n.p.b1 | n.p.b2 | n.p.b3) <= 2);
+            if (qdraw) {
+              v = 0;
+              state = S_RETURN;
+              wsync();
+              continue;
+            }
+            const s32 e = wave_sum_i(eval_square(n.p, lane)) + eval_bishop_pair(n.p);
+            const s32 ev0 = n.p.stm ? -e : e;
+            if (ev0 >= n.beta || sd >= MAXPLY) {
+              if (sd >= MAXPLY) c_maxply++;
+              v = ev0;
+              state = S_RETURN;
+              wsync();
+              continue;
+            }
+          }
+          Legal L;
+          analyse(n.p, L);
           // move generation, lane = square; a wave prefix sum compacts the lists
           u64 t = 0;
           if ((L.own >> lane) & 1) t = piece_targets(n.p, L, lane, n.qs);

@@ -1,0 +1,107 @@
+"""Learned move ordering: record generation, training and evaluation.
+
+Pipeline (all on the GPU through the C-ABI):
+  1. the plain search (MVV-LVA / killer / TT-move ordering, the reference's
+     scheme, src/zeta.cl:2059-2104) runs on seeded playout positions with
+     pair logging on: at every interior cut node the move that caused the
+     cutoff is paired with each move searched before it;
+  2. the 908-256-32-1 network is trained on those pairs with the pairwise
+     ranking loss (BF16 MFMA forward/backward, Adam);
+  3. held-out positions are searched with plain and with learned ordering;
+     the root scores must be identical (alpha-beta's value does not depend on
+     move order) and the node counts / times are compared.
+"""
+import json
+import time
+
+import numpy as np
+
+from . import (MLP_NPARAMS, mlp_get, mlp_init, mlp_set, mlp_train, playouts, records_clear, records_count,
+               records_reserve, search, search_counters)
+
+
+def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, chunk=4096):
+    """Log training pairs from plain searches of n playout positions (20-60 plies)."""
+    records_reserve(capacity)
+    records_clear()
+    pos = playouts(seed, n, 20, 60)
+    t = time.perf_counter()
+    for i in range(0, n, chunk):
+        search(pos[i:i + chunk], depth, workers=workers, tt_log2=26, clear_tt=(i == 0), log_records=True)
+    return {"positions": n, "depth": depth, "pairs": records_count(), "seconds": time.perf_counter() - t}
+
+
+def train(steps=3000, pairs_per_step=2048, lr=1e-3, seed=1, init_seed=1):
+    mlp_init(init_seed)
+    t = time.perf_counter()
+    losses = mlp_train(steps, pairs_per_step, lr, seed)
+    return {"steps": steps, "pairs_per_step": pairs_per_step, "lr": lr, "seconds": time.perf_counter() - t,
+            "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())}
+
+
+def evaluate(seed=40, n=1024, depth=7, workers=1, mlp_min_depth=2, tt_log2=26):
+    """Plain vs learned ordering on held-out positions; identical root scores required."""
+    pos = playouts(seed, n, 20, 60)
+    out = {}
+    res = {}
+    for mode in ("plain", "learned"):
+        t = time.perf_counter()
+        r = search(pos, depth, workers=workers, tt_log2=tt_log2, clear_tt=True, use_mlp=(mode == "learned"),
+                   mlp_min_depth=mlp_min_depth)
+        dt = time.perf_counter() - t
+        res[mode] = r
+        out[mode] = {"seconds": dt, "nodes": int(r["nodes"].sum()), **search_counters()}
+    same = bool((res["plain"]["score"] == res["learned"]["score"]).all())
+    out["identical_root_scores"] = same
+    out["node_reduction"] = 1.0 - out["learned"]["nodes"] / max(out["plain"]["nodes"], 1)
+    out["time_reduction"] = 1.0 - out["learned"]["seconds"] / max(out["plain"]["seconds"], 1e-9)
+    out["config"] = {"positions": n, "seed": seed, "depth": depth, "workers": workers,
+                     "mlp_min_depth": mlp_min_depth, "tt_log2": tt_log2}
+    out["scores"] = res["plain"]["score"].tolist()
+    return out
+
+
+def save(path):
+    np.savez_compressed(path, params=mlp_get())
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        p = z["params"]
+    if p.size != MLP_NPARAMS:
+        raise ValueError("checkpoint has %d parameters, expected %d" % (p.size, MLP_NPARAMS))
+    mlp_set(p)
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser(description="learned move ordering: generate, train, evaluate")
+    ap.add_argument("--train-positions", type=int, default=20000)
+    ap.add_argument("--train-depth", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--eval-positions", type=int, default=1024)
+    ap.add_argument("--eval-depth", type=int, default=7)
+    ap.add_argument("--eval-workers", type=int, default=1)
+    ap.add_argument("--mlp-min-depth", type=int, default=2)
+    ap.add_argument("--checkpoint", default=None)
+    ap.add_argument("--load", default=None)
+    args = ap.parse_args()
+    rep = {}
+    if args.load:
+        load(args.load)
+    else:
+        rep["records"] = generate_records(n=args.train_positions, depth=args.train_depth)
+        print(json.dumps(rep["records"]), flush=True)
+        rep["training"] = train(steps=args.steps)
+        print(json.dumps(rep["training"]), flush=True)
+        if args.checkpoint:
+            save(args.checkpoint)
+    ev = evaluate(n=args.eval_positions, depth=args.eval_depth, workers=args.eval_workers,
+                  mlp_min_depth=args.mlp_min_depth)
+    ev.pop("scores")
+    rep["evaluation"] = ev
+    print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()
