@@ -27,7 +27,9 @@ extern "C" {
 #define ZETA_EHIP (-3)     /* HIP runtime error */
 #define ZETA_ENOMEM (-4)   /* device allocation failed or budget exceeded */
 #define ZETA_ESTATE (-5)   /* library not initialised / wrong state */
-#define ZETA_EBUSY (-6)    /* serving queue full */
+#define ZETA_EBUSY (-6)    /* not admitted now (serving slots / memory budget), or the device is serving */
+#define ZETA_ECANCELED (-7) /* serving request cancelled (zeta_serve_cancel / stop / watchdog) */
+#define ZETA_PENDING 1      /* zeta_serve_poll: not finished yet */
 
 /* quad-bitboard slots (src/types.h:60-66) */
 #define ZETA_QBBBLACK 0 /* black pieces */
@@ -115,6 +117,63 @@ int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, 
  * [6] nodes ordered by the network, [7] training pairs logged.
  * Each probe / store moves one 16-byte entry. */
 int zeta_search_counters(uint64_t *out, int n);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous batching of search requests.  No single counterpart in the
+ * reference, whose host runs one root per cl_run_alphabeta launch
+ * (src/search.c:77-198); each request is a rootsearch (fixed depth,
+ * iterative deepening, node semantics and scores exactly as
+ * zeta_search_batch).  zeta_serve_start launches a persistent grid of
+ * wavefront workers that owns the device until zeta_serve_stop (the other
+ * compute entry points return ZETA_EBUSY meanwhile).  Requests live in
+ * request slots in pinned host memory; an idle worker claims the next
+ * submitted request, and workers with nothing to claim join running requests
+ * (ABDADA helpers, up to the request's worker limit), so requests that
+ * arrive while others run start as soon as a worker frees up and share the
+ * device with them.  All requests share one transposition table, whose
+ * entries cannot change a root score: each result is the same alone and in
+ * any mix.
+ * Admission: a request costs workers x (one worker's LDS stack, MAXPLY plies
+ * + move stack) + the TT share (TT bytes / slots); zeta_serve_submit returns
+ * ZETA_EBUSY while the slots or mem_budget cannot take it (retry after
+ * zeta_serve_reap frees some), ZETA_ENOMEM if it could never be admitted.
+ * Every host call renews a heartbeat; if the host makes no serving call for
+ * watchdog_ms the grid cancels its requests and exits. */
+typedef struct {
+  int slots;            /* request slots (requests in flight), 1..4096 */
+  int workers;          /* default and maximum workers per request, 1..64 */
+  int tt_log2;          /* shared TT size 2^tt_log2 x 16 B; -1 = zeta_config.tt_log2; 0 = none */
+  int use_mlp;          /* requests with ZETA_REQ_LEARNED use the loaded network */
+  int mlp_min_depth;
+  int grid;             /* persistent workers, 0 = fill the device */
+  int watchdog_ms;      /* 0 = 10000 */
+  int reserved;
+  uint64_t mem_budget;  /* admission budget in bytes, 0 = no limit */
+} zeta_serve_opts;
+#define ZETA_REQ_LEARNED 1 /* learned move ordering (plain ordering when no network is loaded) */
+typedef struct {
+  zeta_pos pos;
+  int depth;            /* 1..63 */
+  int workers;          /* 0 = zeta_serve_opts.workers */
+  int flags;            /* ZETA_REQ_* */
+  int reserved;
+} zeta_serve_request;
+typedef struct {
+  uint64_t ticket;
+  zeta_result res;      /* status ZETA_OK, ZETA_ENOMEM (move stack overflow) or ZETA_ECANCELED */
+} zeta_serve_done;
+int zeta_serve_start(const zeta_serve_opts *opts);
+/* bytes a request with this many workers charges against mem_budget */
+int zeta_serve_cost(int workers, uint64_t *bytes);
+int zeta_serve_submit(const zeta_serve_request *req, uint64_t *ticket);
+/* ZETA_OK and *res when the request has finished (its slot is freed), else ZETA_PENDING */
+int zeta_serve_poll(uint64_t ticket, zeta_result *res);
+/* all finished requests not yet collected, at most max; *n = how many */
+int zeta_serve_reap(zeta_serve_done *out, int max, int *n);
+int zeta_serve_cancel(uint64_t ticket);
+/* cancels what is still running, waits for the grid to exit, frees the
+ * slots; counters as zeta_search_counters; total_nodes may be NULL */
+int zeta_serve_stop(uint64_t *total_nodes);
 
 /* ------------------------------------------------ learned move ordering
  * No counterpart in the reference (its ordering is fixed scores,

@@ -51,6 +51,29 @@ RESULT_DTYPE = np.dtype([("score", "<i4"), ("bestmove", "<u4"), ("nodes", "<u8")
                          ("status", "<i4")])
 
 
+class ServeOpts(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("workers", ctypes.c_int), ("tt_log2", ctypes.c_int),
+                ("use_mlp", ctypes.c_int), ("mlp_min_depth", ctypes.c_int), ("grid", ctypes.c_int),
+                ("watchdog_ms", ctypes.c_int), ("reserved", ctypes.c_int), ("mem_budget", ctypes.c_uint64)]
+
+
+class _Pos(ctypes.Structure):
+    _fields_ = [("b", ctypes.c_uint64 * 7), ("stm", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("pos", _Pos), ("depth", ctypes.c_int), ("workers", ctypes.c_int), ("flags", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
+SERVE_DONE_DTYPE = np.dtype([("ticket", "<u8"), ("score", "<i4"), ("bestmove", "<u4"), ("nodes", "<u8"),
+                             ("depth", "<i4"), ("status", "<i4")])
+ZETA_EBUSY = -6
+ZETA_ECANCELED = -7
+ZETA_PENDING = 1
+ZETA_REQ_LEARNED = 1
+
+
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("tt_log2", ctypes.c_int),
                 ("frontier_budget", ctypes.c_uint64), ("workers", ctypes.c_int),
@@ -89,6 +112,13 @@ def lib():
         L.zeta_dev_free.argtypes = [vp]
         L.zeta_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
         L.zeta_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
+        L.zeta_serve_start.argtypes = [ctypes.POINTER(ServeOpts)]
+        L.zeta_serve_cost.argtypes = [i32, vp]
+        L.zeta_serve_submit.argtypes = [ctypes.POINTER(ServeRequest), vp]
+        L.zeta_serve_poll.argtypes = [u64, vp]
+        L.zeta_serve_reap.argtypes = [vp, i32, vp]
+        L.zeta_serve_cancel.argtypes = [u64]
+        L.zeta_serve_stop.argtypes = [vp]
         for f in ("zeta_init", "zeta_shutdown", "zeta_last_error", "zeta_device_info",
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
                   "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
@@ -96,7 +126,8 @@ def lib():
                   "zeta_search_batch", "zeta_search_counters", "zeta_mlp_init", "zeta_mlp_set",
                   "zeta_mlp_get", "zeta_mlp_grad", "zeta_mlp_train", "zeta_records_reserve",
                   "zeta_records_count", "zeta_records_clear", "zeta_records_get", "zeta_records_put",
-                  "zeta_mlp_score"):
+                  "zeta_mlp_score", "zeta_serve_start", "zeta_serve_cost", "zeta_serve_submit",
+                  "zeta_serve_poll", "zeta_serve_reap", "zeta_serve_cancel", "zeta_serve_stop"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -330,3 +361,77 @@ def mlp_score(pos, moves_per_pos):
     out = np.zeros(len(mv), dtype=np.float32)
     _check(lib().zeta_mlp_score(_ptr(pos), len(pos), _ptr(mv), _ptr(offs), _ptr(out)), "zeta_mlp_score")
     return [out[offs[i]:offs[i + 1]] for i in range(len(pos))]
+
+
+# ---------------------------------------------------------------- serving
+class Server:
+    """Continuous-batching search service on the device (zeta_serve_*).
+
+    A persistent grid of wavefront workers takes requests from slots in
+    pinned host memory as they are submitted; idle workers help running
+    requests.  Each request is a fixed-depth root search with the same
+    result as search() on that position (a shared TT cannot change a root
+    score).  While a Server is running the other compute calls raise
+    (ZETA_EBUSY).
+    """
+
+    def __init__(self, slots=256, workers=8, tt_log2=24, use_mlp=False, mlp_min_depth=2, grid=0,
+                 watchdog_ms=10000, mem_budget=0):
+        opts = ServeOpts(slots, workers, tt_log2, int(use_mlp), mlp_min_depth, grid, watchdog_ms, 0, mem_budget)
+        _check(lib().zeta_serve_start(ctypes.byref(opts)), "zeta_serve_start")
+        self.slots = slots
+        self._done = np.zeros(slots, dtype=SERVE_DONE_DTYPE)
+        self.running = True
+
+    def cost(self, workers=0):
+        out = ctypes.c_uint64()
+        _check(lib().zeta_serve_cost(workers, ctypes.byref(out)), "zeta_serve_cost")
+        return out.value
+
+    def submit(self, pos, depth, workers=0, learned=False):
+        """Ticket of the admitted request, or None when it cannot be admitted now."""
+        p = as_positions(pos)
+        req = ServeRequest()
+        ctypes.memmove(ctypes.byref(req.pos), p.ctypes.data, 64)
+        req.depth = depth
+        req.workers = workers
+        req.flags = ZETA_REQ_LEARNED if learned else 0
+        t = ctypes.c_uint64()
+        rc = lib().zeta_serve_submit(ctypes.byref(req), ctypes.byref(t))
+        if rc == ZETA_EBUSY:
+            return None
+        _check(rc, "zeta_serve_submit")
+        return t.value
+
+    def poll(self, ticket):
+        """Result record of a finished request, or None while it runs."""
+        out = np.zeros(1, dtype=RESULT_DTYPE)
+        rc = lib().zeta_serve_poll(ticket, _ptr(out))
+        if rc == ZETA_PENDING:
+            return None
+        _check(rc, "zeta_serve_poll")
+        return out[0]
+
+    def reap(self):
+        """All finished requests not yet collected (SERVE_DONE_DTYPE array)."""
+        n = ctypes.c_int()
+        _check(lib().zeta_serve_reap(_ptr(self._done), self.slots, ctypes.byref(n)), "zeta_serve_reap")
+        return self._done[:n.value].copy()
+
+    def cancel(self, ticket):
+        _check(lib().zeta_serve_cancel(ticket), "zeta_serve_cancel")
+
+    def stop(self):
+        """Stop the grid; returns the nodes of all collected and finished requests."""
+        if not self.running:
+            return 0
+        tot = ctypes.c_uint64()
+        self.running = False
+        _check(lib().zeta_serve_stop(ctypes.byref(tot)), "zeta_serve_stop")
+        return tot.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()

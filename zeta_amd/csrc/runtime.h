@@ -42,6 +42,7 @@ struct Ctx {
   std::string arch;
   int cu_count = 0;
   bool timing = false;
+  bool serving = false;  // a persistent serving grid owns the stream (zeta_serve_start .. stop)
   // timing
   struct Pending {
     int id;
@@ -56,7 +57,8 @@ struct Ctx {
 Ctx &ctx();
 int fail(int code, const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
-int require_up();
+int require_up();       // initialised and the stream free (ZETA_EBUSY while serving)
+int require_init();     // initialised
 int ensure(DevBuf &b, size_t bytes);  // grow-only device buffer
 void release(DevBuf &b);
 

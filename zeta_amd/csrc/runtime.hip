@@ -28,8 +28,14 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
   return fail(ZETA_EHIP, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
 }
 
+int require_init() {
+  if (!g_ctx.up) return fail(ZETA_ESTATE, "library not initialised (call zeta_init)");
+  return ZETA_OK;
+}
+
 int require_up() {
   if (!g_ctx.up) return fail(ZETA_ESTATE, "library not initialised (call zeta_init)");
+  if (g_ctx.serving) return fail(ZETA_EBUSY, "the device is serving (call zeta_serve_stop first)");
   return ZETA_OK;
 }
 
@@ -152,6 +158,7 @@ void mlp_release();
 extern "C" int zeta_shutdown(void) {
   Ctx &c = ctx();
   if (!c.up) return ZETA_OK;
+  if (c.serving) (void)zeta_serve_stop(nullptr);
   (void)hipStreamSynchronize(c.stream);
   time_collect();
   perft_release();
@@ -172,7 +179,7 @@ extern "C" int zeta_last_error(char *buf, size_t n) {
 }
 
 extern "C" int zeta_device_info(char *buf, size_t n) {
-  int r = require_up();
+  int r = require_init();
   if (r) return r;
   if (!buf || n == 0) return ZETA_EINVAL;
   snprintf(buf, n, "%s cu=%d", ctx().arch.c_str(), ctx().cu_count);
@@ -180,7 +187,7 @@ extern "C" int zeta_device_info(char *buf, size_t n) {
 }
 
 extern "C" int zeta_set_timing(int on) {
-  int r = require_up();
+  int r = require_init();
   if (r) return r;
   ctx().timing = on != 0;
   return ZETA_OK;

@@ -54,6 +54,53 @@ constexpr int POLL_EVERY = 64;
 
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte TT entry
 
+// ---- serving (zeta_serve_*): the host publishes requests in coherent pinned
+// host memory; a persistent grid of workers claims them in ticket order,
+// helps running ones (ABDADA) when idle, and writes results back there.
+struct ServeReq {  // 80 B
+  u64 pos[8];      // zeta_pos
+  s32 depth;
+  s32 flags;       // bit 0: learned ordering
+  s32 workers;     // at most this many workers on the request
+  s32 pad;
+};
+struct ServeSlot {  // 128 B, pinned host memory
+  ServeReq req;     // written by the host before the ticket is published
+  s32 score;        // results, written by the device
+  u32 bestmove;
+  u64 nodes;
+  u64 ticket;       // ticket + 1 of the result
+  s32 depth_done;
+  s32 status;       // SV_DONE / SV_FAILED / SV_CANCELLED
+  u32 state;        // 0 -> 1 when the result is complete (device store, host clears)
+  u32 pad[3];
+};
+struct ServeHost {  // pinned host memory header, then u64 queue[qcap], ack[qcap], cancel[nslots]
+  u64 submitted;    // tickets published; ticket t names slot queue[t % qcap]
+  u64 heartbeat;    // bumped by every host call (watchdog)
+  u32 shutdown;
+  u32 cancel_gen;   // bumped when cancel[] changes
+  u32 pad[10];
+};
+struct ServeDev {  // device memory; mirrors of the host words, refreshed by one worker at a time
+  u64 claimed;     // tickets claimed by workers
+  u64 sub;         // mirror of ServeHost::submitted
+  u64 hb;          // last heartbeat seen
+  u64 hb_time;     // wall clock when it last changed (0 = not yet)
+  u64 last_refresh;
+  u32 shut;        // 1 shutdown, 2 watchdog, 3 protocol error
+  u32 poller;      // try-lock of the refresh
+  u32 cancel_gen;
+  u32 hb_xcc;      // XCC of the worker that timed the last heartbeat
+  u64 exit_now;    // diagnostics of a watchdog / protocol exit
+  u64 exit_arg;
+  u32 exit_xcc;
+  u32 pad;
+};
+// slot word (device): ticket+1 << 24 | max workers << 16 | workers << 2 | status
+enum { SV_RUN = 0, SV_DONE = 1, SV_FAILED = 2, SV_CANCELLED = 3 };
+
+// batch counters (device, summed per worker at the end of each root)
 struct SearchArgs {
   const u64 *roots;  // zeta_pos[n]
   int nroots;
@@ -82,12 +129,25 @@ struct SearchArgs {
   u64 *recs;
   u64 *rec_count;
   u64 rec_cap;
+  // serving mode (slots != nullptr)
+  ServeHost *sh;
+  const u64 *squeue;  // host: slot of ticket t at [t % qcap]
+  u64 *sack;          // host: ticket + 1 once claimed
+  const u64 *scancel; // host: ticket + 1 to cancel, per slot
+  ServeSlot *slots;   // host
+  int nslots;
+  int qcap;
+  ServeDev *sv;       // device
+  u64 *sw;            // device slot words
+  ServeReq *dreq;     // device copies of the claimed requests
+  u64 *dcancel;       // device mirror of scancel
+  u64 watchdog_ticks; // wall-clock ticks without a heartbeat before the grid exits (0 = never)
+  u64 refresh_ticks;  // minimum wall-clock ticks between two refreshes of the mirrors
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// batch counters (device, summed per worker at the end of each root)
 enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_MLP_NODES, C_RECORDS, C_NCTR = 8 };
 
 // One ply of a worker's stack in LDS (64 B).  The board itself is not kept:
@@ -412,6 +472,174 @@ __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u3
   }
 }
 
+__device__ __forceinline__ u64 sys_load64(const u64 *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ u32 sys_load32(const u32 *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T> __device__ __forceinline__ T ag_load(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void ag_store(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ bool ag_cas(T *p, T expect, T v) {
+  return __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 sw_tk(u64 v) { return v >> 24; }
+__device__ __forceinline__ int sw_max(u64 v) { return (int)((v >> 16) & 0xFF); }
+__device__ __forceinline__ int sw_w(u64 v) { return (int)((v >> 2) & 0x3FFF); }
+__device__ __forceinline__ int sw_st(u64 v) { return (int)(v & 3); }
+
+// refresh the device mirrors of the host words (lane 0; one worker at a time)
+__device__ __forceinline__ u32 xcc_id() {
+  // HW_REG_XCC_ID (hwreg 20), bits [3:0]
+  return (u32)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 15u;
+}
+
+__device__ void serve_refresh(const SearchArgs &a) {
+  ServeDev *v = a.sv;
+  const u64 now = wall_clock64();
+  if ((long long)(now - ag_load(&v->last_refresh)) < (long long)a.refresh_ticks) return;
+  if (!ag_cas(&v->poller, 0u, 1u)) return;
+  ag_store(&v->last_refresh, now);
+  const u64 sub = sys_load64(&a.sh->submitted);
+  const u64 hb = sys_load64(&a.sh->heartbeat);
+  u32 shut = sys_load32(&a.sh->shutdown) ? 1u : 0u;
+  const u32 cg = sys_load32(&a.sh->cancel_gen);
+  if (hb != ag_load(&v->hb) || ag_load(&v->hb_time) == 0) {
+    ag_store(&v->hb, hb);
+    ag_store(&v->hb_time, now);
+    ag_store(&v->hb_xcc, xcc_id());
+  } else if (a.watchdog_ticks && (long long)(now - ag_load(&v->hb_time)) > (long long)a.watchdog_ticks) {
+    shut = 2;
+    ag_store(&v->exit_now, now);
+    ag_store(&v->exit_arg, ag_load(&v->hb_time));
+    ag_store(&v->exit_xcc, xcc_id() | (ag_load(&v->hb_xcc) << 8));
+  }
+  if (cg != ag_load(&v->cancel_gen)) {
+    for (int s = 0; s < a.nslots; s++) ag_store(&a.dcancel[s], sys_load64(&a.scancel[s]));
+    ag_store(&v->cancel_gen, cg);
+  }
+  ag_store(&v->sub, sub);
+  if (shut && !ag_load(&v->shut)) ag_store(&v->shut, shut);
+  ag_store(&v->poller, 0u);
+}
+
+// serving: claim the next published ticket, else join the running request
+// with the fewest workers; returns its slot and ticket+1, or -1 on shutdown
+__device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
+  for (;;) {
+    int res = -2;
+    u64 tk = 0;
+    if (lane == 0) {
+      serve_refresh(a);
+      if (ag_load(&a.sv->shut)) {
+        res = -1;
+      } else {
+        const u64 c = ag_load(&a.sv->claimed);
+        if (c < ag_load(&a.sv->sub) && ag_cas(&a.sv->claimed, c, c + 1)) {
+          const int qi = (int)(c % (u64)a.qcap);
+          const u64 s = sys_load64(&a.squeue[qi]);
+          if (s >= (u64)a.nslots) {
+            ag_store(&a.sv->exit_now, c);
+            ag_store(&a.sv->exit_arg, s);
+            ag_store(&a.sv->shut, 3u);  // the host broke the protocol: stop serving
+            res = -1;
+          } else {
+            ServeReq &d = a.dreq[s];
+            const ServeReq &h = a.slots[s].req;
+            for (int k = 0; k < 8; k++) d.pos[k] = h.pos[k];
+            d.depth = h.depth;
+            d.flags = h.flags;
+            d.workers = h.workers;
+            __hip_atomic_store(&a.sack[qi], c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            ag_store(&a.nodes[s], (u64)0);
+            ag_store(&a.depth_done[s], 0);
+            const u64 wmax = (u64)(d.workers < 1 ? 1 : (d.workers > 255 ? 255 : d.workers));
+            ag_store(&a.sw[s], (u64)(((c + 1) << 24) | (wmax << 16) | (u64)4 | (u64)SV_RUN));
+            res = (int)s;
+            tk = c + 1;
+          }
+        }
+      }
+    }
+    res = uf(res);
+    if (res != -2) {
+      tk_out = uf(tk);
+      return res;
+    }
+    // no new request: the running request with the fewest workers below its maximum
+    u32 bestkey = 0;
+    for (int base = 0; base < a.nslots; base += 64) {
+      const int s = base + lane;
+      if (s < a.nslots) {
+        const u64 v = ag_load(&a.sw[s]);
+        const int w = sw_w(v);
+        if (sw_tk(v) && sw_st(v) == SV_RUN && w >= 1 && w < sw_max(v))
+          bestkey = max(bestkey, ((u32)(255 - w) << 16) | (u32)(0xFFFF - s));
+      }
+    }
+    bestkey = wave_max_u32(bestkey);
+    if (bestkey) {
+      const int s = 0xFFFF - (int)(bestkey & 0xFFFF);
+      int got = -1;
+      if (lane == 0) {
+        const u64 v = ag_load(&a.sw[s]);
+        if (sw_tk(v) && sw_st(v) == SV_RUN && sw_w(v) >= 1 && sw_w(v) < sw_max(v) && ag_cas(&a.sw[s], v, v + 4)) {
+          got = s;
+          tk = sw_tk(v);
+        }
+      }
+      got = uf(got);
+      if (got >= 0) {
+        tk_out = uf(tk);
+        return got;
+      }
+    } else {
+      __builtin_amdgcn_s_sleep(32);
+    }
+  }
+}
+
+// serving: abort test at the periodic poll
+__device__ __forceinline__ bool serve_abort(const SearchArgs &a, int r, u64 tk) {
+  serve_refresh(a);
+  const u64 v = ag_load(&a.sw[r]);
+  return sw_tk(v) != tk || sw_st(v) != SV_RUN || ag_load(&a.dcancel[r]) == tk || ag_load(&a.sv->shut) != 0;
+}
+
+// serving: leave the request; the worker that completes it (or the last one
+// to leave an aborted request) writes the result to the host slot
+__device__ void serve_leave(const SearchArgs &a, int r, u64 tk, bool aborted, bool failed, s32 score, u32 move) {
+  for (;;) {
+    const u64 v = ag_load(&a.sw[r]);
+    if (sw_tk(v) != tk) return;  // the slot has moved on
+    u64 nv = v - 4;
+    int pub = -1;
+    if (sw_st(v) == SV_RUN) {
+      if (failed) pub = SV_FAILED;
+      else if (!aborted) pub = SV_DONE;
+      else if (sw_w(v) == 1) pub = SV_CANCELLED;
+      if (pub >= 0) nv = (nv & ~3ULL) | (u64)pub;
+    }
+    if (!ag_cas(&a.sw[r], v, nv)) continue;
+    if (pub >= 0) {
+      ServeSlot &sl = a.slots[r];
+      sl.score = pub == SV_DONE ? score : 0;
+      sl.bestmove = pub == SV_DONE ? move : 0u;
+      sl.nodes = ag_load(&a.nodes[r]);
+      sl.ticket = tk;
+      sl.depth_done = ag_load(&a.depth_done[r]);
+      sl.status = pub;
+      __hip_atomic_store(&sl.state, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return;
+  }
+}
+
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_search(SearchArgs a) {
   __shared__ Ply ply[MAXPLY + 2];
   __shared__ u32 mmove[MSTACK];
@@ -423,14 +651,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   const int lane = threadIdx.x;
   const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
 
+  const bool serving = a.slots != nullptr;
   for (;;) {
-    int rr = -1;
-    if (lane == 0) rr = *a.abort_flag ? -1 : pick_root(a, hstart);
-    const int r = uf(rr);
+    int r;
+    u64 ticket = 0;
+    if (serving) {
+      r = serve_pick(a, lane, ticket);
+    } else {
+      int rr = -1;
+      if (lane == 0) rr = *a.abort_flag ? -1 : pick_root(a, hstart);
+      r = uf(rr);
+    }
     if (r < 0) return;
+    int depth_r = a.depth;
+    bool mlp_r = a.use_mlp != 0;
+    if (serving) {
+      depth_r = uf(ag_load(&a.dreq[r].depth));
+      mlp_r = mlp_r && (uf(ag_load(&a.dreq[r].flags)) & 1);
+      if (depth_r < 1) depth_r = 1;
+      if (depth_r > MAXPLY - 1) depth_r = MAXPLY - 1;
+    }
 
     // ---------------------------------------------------------------- root set-up
-    const u64 *q = a.roots + (size_t)r * 8;
+    const u64 *q = serving ? a.dreq[r].pos : a.roots + (size_t)r * 8;
     Pos root;
     root.b0 = q[0];
     root.b1 = q[1];
@@ -450,7 +693,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     s32 final_score = 0;
     u32 final_move = 0;
 
-    for (int id = 1; id <= a.depth && !aborted && !failed; id++) {
+    for (int id = 1; id <= depth_r && !aborted && !failed; id++) {
       Node n;
       n.p = root;
       n.hash = roothash;
@@ -484,8 +727,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             poll = 0;
             u32 d = 0;
             if (lane == 0) {
-              d = __atomic_load_n(&a.done[r], __ATOMIC_RELAXED) | __atomic_load_n(a.abort_flag, __ATOMIC_RELAXED);
-              if (a.max_nodes && nodes > a.max_nodes) d = 1;
+              if (serving) {
+                d = serve_abort(a, r, ticket);
+              } else {
+                d = __atomic_load_n(&a.done[r], __ATOMIC_RELAXED) | __atomic_load_n(a.abort_flag, __ATOMIC_RELAXED);
+                if (a.max_nodes && nodes > a.max_nodes) d = 1;
+              }
             }
             if (uf(d)) {
               aborted = true;
@@ -560,7 +807,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               k++;
             }
           }
-          if (a.use_mlp && !n.qs && n.depth >= a.mlp_min_depth && total > 1) {
+          if (mlp_r && !n.qs && n.depth >= a.mlp_min_depth && total > 1) {
             wsync();
             mlp_score_moves(a, n.p, n.mbeg, total, lane, hpos, mmove, mscore);
             c_mlp++;
@@ -756,12 +1003,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       }
       final_score = (rootscore == INF || rootscore == -INF) ? 0 : rootscore;  // src/search.c:137
       final_move = rootmove;
-      if (lane == 0) atomicMax(&a.depth_done[r], id);
+      if (lane == 0 && (!serving || sw_tk(ag_load(&a.sw[r])) == ticket)) atomicMax(&a.depth_done[r], id);
     }
 
     // ---------------------------------------------------------------- publish
     if (lane == 0) {
-      atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
+      if (!serving || sw_tk(ag_load(&a.sw[r])) == ticket)
+        atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
       atomicAdd((unsigned long long *)&a.ctr[C_TT_PROBE], (unsigned long long)c_probe);
       atomicAdd((unsigned long long *)&a.ctr[C_TT_HIT], (unsigned long long)c_hit);
       atomicAdd((unsigned long long *)&a.ctr[C_TT_CUT], (unsigned long long)c_cut);
@@ -770,12 +1018,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       atomicAdd((unsigned long long *)&a.ctr[C_MAXPLY], (unsigned long long)c_maxply);
       atomicAdd((unsigned long long *)&a.ctr[C_MLP_NODES], (unsigned long long)c_mlp);
       atomicAdd((unsigned long long *)&a.ctr[C_RECORDS], (unsigned long long)c_rec);
-      if (!aborted && !failed && atomicCAS(&a.done[r], 0u, 1u) == 0u) {
-        a.score[r] = final_score;
-        a.bestmove[r] = final_move;
+      if (serving) {
+        serve_leave(a, r, ticket, aborted, failed, final_score, final_move);
+      } else {
+        if (!aborted && !failed && atomicCAS(&a.done[r], 0u, 1u) == 0u) {
+          a.score[r] = final_score;
+          a.bestmove[r] = final_move;
+        }
+        if (failed) atomicCAS(&a.done[r], 0u, 2u);
+        atomicSub(&a.nworkers[r], 1);
       }
-      if (failed) atomicCAS(&a.done[r], 0u, 2u);
-      atomicSub(&a.nworkers[r], 1);
     }
     wsync();
   }
@@ -908,6 +1160,19 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.recs = nullptr;
   sa.rec_count = nullptr;
   sa.rec_cap = 0;
+  sa.sh = nullptr;
+  sa.squeue = nullptr;
+  sa.sack = nullptr;
+  sa.scancel = nullptr;
+  sa.slots = nullptr;
+  sa.nslots = 0;
+  sa.qcap = 0;
+  sa.sv = nullptr;
+  sa.sw = nullptr;
+  sa.dreq = nullptr;
+  sa.dcancel = nullptr;
+  sa.watchdog_ticks = 0;
+  sa.refresh_ticks = 0;
   if (opts->log_records) {
     u64 cap = 0, *cnt = nullptr;
     void *recs = records_buffer(&cap, &cnt);
@@ -986,5 +1251,343 @@ extern "C" int zeta_mlp_score(const zeta_pos *pos, int npos, const uint32_t *mov
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   zr::release(dp); zr::release(dm); zr::release(doff); zr::release(dout);
   if (e != hipSuccess) return zr::hip_fail(e, "zeta_mlp_score", __FILE__, __LINE__);
+  return ZETA_OK;
+}
+
+// ============================================================== serving
+namespace {
+
+struct ServeState {
+  bool on = false;
+  int nslots = 0, qcap = 0, workers = 1;
+  void *pinned = nullptr;  // ServeHost + queue + ack + cancel + slots (hipHostMalloc, coherent)
+  ServeHost *sh = nullptr;
+  u64 *queue = nullptr, *ack = nullptr, *cancel = nullptr;
+  ServeSlot *slots = nullptr;
+  zr::DevBuf sv, sw, dreq, dcancel, nodes, depth_done, busy, ctr, err;
+  std::vector<u64> slot_ticket;  // ticket + 1 held by the slot, 0 = free
+  std::vector<u64> slot_cost;
+  std::vector<int> free_slots;
+  u64 next_ticket = 0;
+  u64 budget = 0, in_use = 0, tt_bytes = 0, worker_bytes = 0;
+  u64 total_nodes = 0;
+  bool use_mlp = false;
+};
+ServeState g_sv;
+
+inline void sv_heartbeat() {
+  __atomic_fetch_add(&g_sv.sh->heartbeat, 1ULL, __ATOMIC_RELEASE);
+}
+
+u64 sv_cost(int workers) {
+  return (u64)workers * g_sv.worker_bytes + (g_sv.nslots ? g_sv.tt_bytes / (u64)g_sv.nslots : 0);
+}
+
+int sv_require() {
+  int r = zr::require_init();
+  if (r) return r;
+  if (!g_sv.on) return zr::fail(ZETA_ESTATE, "serving not started (zeta_serve_start)");
+  return ZETA_OK;
+}
+
+// the grid has exited on its own (watchdog or protocol error): report it
+int sv_check_alive() {
+  hipError_t q = hipStreamQuery(zr::ctx().stream);
+  if (q == hipErrorNotReady) return ZETA_OK;
+  if (q != hipSuccess) return zr::hip_fail(q, "serving grid", __FILE__, __LINE__);
+  ServeDev d;
+  memset(&d, 0, sizeof(d));
+  (void)hipMemcpy(&d, g_sv.sv.p, sizeof(d), hipMemcpyDeviceToHost);
+  return zr::fail(ZETA_ESTATE,
+                  "serving grid has exited (reason %u: 2 watchdog, 3 host protocol error; now/ticket %llu, "
+                  "arg %llu, xcc %#x, claimed %llu, mirror %llu); call zeta_serve_stop",
+                  d.shut, (unsigned long long)d.exit_now, (unsigned long long)d.exit_arg, d.exit_xcc,
+                  (unsigned long long)d.claimed, (unsigned long long)d.sub);
+}
+
+void sv_take(int s, zeta_result *res) {
+  ServeSlot &sl = g_sv.slots[s];
+  res->score = sl.score;
+  res->bestmove = sl.bestmove;
+  res->nodes = sl.nodes;
+  res->depth = sl.depth_done;
+  res->status = sl.status == SV_DONE ? ZETA_OK : (sl.status == SV_FAILED ? ZETA_ENOMEM : ZETA_ECANCELED);
+  g_sv.total_nodes += sl.nodes;
+  __atomic_store_n(&sl.state, 0u, __ATOMIC_RELEASE);
+  g_sv.in_use -= g_sv.slot_cost[s];
+  g_sv.slot_cost[s] = 0;
+  g_sv.slot_ticket[s] = 0;
+  g_sv.free_slots.push_back(s);
+}
+
+void sv_free_all() {
+  zr::release(g_sv.sv); zr::release(g_sv.sw); zr::release(g_sv.dreq); zr::release(g_sv.dcancel);
+  zr::release(g_sv.nodes); zr::release(g_sv.depth_done); zr::release(g_sv.busy); zr::release(g_sv.ctr);
+  zr::release(g_sv.err);
+  if (g_sv.pinned) (void)hipHostFree(g_sv.pinned);
+  g_sv.pinned = nullptr;
+  g_sv.sh = nullptr;
+  g_sv.slots = nullptr;
+  g_sv.on = false;
+  zr::ctx().serving = false;
+}
+
+}  // namespace
+
+extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
+  int r = zr::require_up();
+  if (r) return r;
+  if (!o) return zr::fail(ZETA_EINVAL, "zeta_serve_start: null options");
+  if (o->slots < 1 || o->slots > 4096) return zr::fail(ZETA_EINVAL, "zeta_serve_start: slots %d not in 1..4096", o->slots);
+  if (o->workers < 1 || o->workers > 64)
+    return zr::fail(ZETA_EINVAL, "zeta_serve_start: workers %d not in 1..64", o->workers);
+  const int tt_log2 = o->tt_log2 < 0 ? zr::ctx().cfg.tt_log2 : o->tt_log2;
+  if (tt_log2 > 34) return zr::fail(ZETA_EINVAL, "zeta_serve_start: tt_log2 %d too large", tt_log2);
+  const zm::Weights *w = nullptr;
+  if (o->use_mlp) w = mlp_weights();  // no network loaded: plain ordering for every request
+  hipStream_t st = zr::ctx().stream;
+  ZCHECK(hipStreamSynchronize(st));
+
+  ServeState &S = g_sv;
+  S = ServeState();
+  S.nslots = o->slots;
+  S.qcap = 2 * o->slots;
+  S.workers = o->workers;
+  S.budget = o->mem_budget;
+  S.use_mlp = w != nullptr;
+  hipFuncAttributes fa;
+  ZCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_search)));
+  S.worker_bytes = (u64)fa.sharedSizeBytes;
+  const u64 tt_entries = tt_log2 > 0 ? (1ULL << tt_log2) : 0;
+  S.tt_bytes = tt_entries * 16;
+
+  const size_t hbytes = sizeof(ServeHost) + (size_t)(2 * S.qcap + S.nslots) * 8 + (size_t)S.nslots * sizeof(ServeSlot);
+  hipError_t e = hipHostMalloc(&S.pinned, hbytes, hipHostMallocCoherent | hipHostMallocMapped);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    S.pinned = nullptr;
+    return zr::fail(ZETA_ENOMEM, "zeta_serve_start: hipHostMalloc of %zu bytes failed (%s)", hbytes, hipGetErrorString(e));
+  }
+  memset(S.pinned, 0, hbytes);
+  char *hp = (char *)S.pinned;
+  S.sh = (ServeHost *)hp;
+  S.queue = (u64 *)(hp + sizeof(ServeHost));
+  S.ack = S.queue + S.qcap;
+  S.cancel = S.ack + S.qcap;
+  S.slots = (ServeSlot *)(S.cancel + S.nslots);
+  void *dev_pinned = nullptr;
+  e = hipHostGetDevicePointer(&dev_pinned, S.pinned, 0);
+  if (e != hipSuccess) {
+    sv_free_all();
+    return zr::hip_fail(e, "hipHostGetDevicePointer", __FILE__, __LINE__);
+  }
+  const ptrdiff_t dd = (char *)dev_pinned - (char *)S.pinned;
+  auto dp = [dd](void *p) { return (void *)((char *)p + dd); };
+
+  const u32 busy_entries = 1u << 20;
+  if ((r = zr::ensure(S.sv, sizeof(ServeDev))) || (r = zr::ensure(S.sw, (size_t)S.nslots * 8)) ||
+      (r = zr::ensure(S.dreq, (size_t)S.nslots * sizeof(ServeReq))) || (r = zr::ensure(S.dcancel, (size_t)S.nslots * 8)) ||
+      (r = zr::ensure(S.nodes, (size_t)S.nslots * 8)) || (r = zr::ensure(S.depth_done, (size_t)S.nslots * 4)) ||
+      (r = zr::ensure(S.busy, (size_t)busy_entries * 4)) || (r = zr::ensure(S.ctr, 64)) || (r = zr::ensure(S.err, 64))) {
+    sv_free_all();
+    return r;
+  }
+  if (tt_entries && ((r = zr::ensure(g_ss.tt, tt_entries * 16)))) {
+    sv_free_all();
+    return r;
+  }
+#define SV_CHECK(call)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      int rc_ = zr::hip_fail(e_, #call, __FILE__, __LINE__);                    \
+      sv_free_all();                                                            \
+      return rc_;                                                               \
+    }                                                                           \
+  } while (0)
+  if (tt_entries && g_ss.tt_entries != tt_entries) {
+    hipLaunchKernelGGL(k_clear_u64, dim3(4096), dim3(256), 0, st, (u64 *)g_ss.tt.p, (size_t)(tt_entries * 2));
+    SV_CHECK(hipGetLastError());
+    g_ss.tt_entries = tt_entries;
+  }
+  SV_CHECK(hipMemsetAsync(S.sv.p, 0, sizeof(ServeDev), st));
+  SV_CHECK(hipMemsetAsync(S.sw.p, 0, (size_t)S.nslots * 8, st));
+  SV_CHECK(hipMemsetAsync(S.dcancel.p, 0, (size_t)S.nslots * 8, st));
+  SV_CHECK(hipMemsetAsync(S.nodes.p, 0, (size_t)S.nslots * 8, st));
+  SV_CHECK(hipMemsetAsync(S.depth_done.p, 0, (size_t)S.nslots * 4, st));
+  SV_CHECK(hipMemsetAsync(S.busy.p, 0, (size_t)busy_entries * 4, st));
+  SV_CHECK(hipMemsetAsync(S.ctr.p, 0, 64, st));
+  SV_CHECK(hipMemsetAsync(S.err.p, 0, 64, st));
+  SV_CHECK(hipStreamSynchronize(st));
+
+  int rate_khz = 0;
+  SV_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, zr::ctx().device));
+  if (rate_khz <= 0) rate_khz = 100000;
+  const int wd_ms = o->watchdog_ms > 0 ? o->watchdog_ms : 10000;
+
+  int per_cu = 0;
+  SV_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search, 64, 0));
+  if (per_cu < 1) per_cu = 1;
+  const int cap = per_cu * zr::ctx().cu_count;
+  const int grid = o->grid > 0 && o->grid < cap ? o->grid : cap;
+
+  SearchArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.workers = S.workers;
+  sa.tt = (u64 *)g_ss.tt.p;
+  sa.ttmask = tt_entries ? tt_entries - 1 : 0;
+  sa.busy = (u32 *)S.busy.p;
+  sa.busymask = busy_entries - 1;
+  sa.nodes = (u64 *)S.nodes.p;
+  sa.depth_done = (s32 *)S.depth_done.p;
+  sa.err = (u32 *)S.err.p;
+  sa.ctr = (u64 *)S.ctr.p;
+  sa.use_mlp = S.use_mlp ? 1 : 0;
+  sa.mlp_min_depth = o->mlp_min_depth;
+  if (w) sa.w = *w;
+  sa.sh = (ServeHost *)dp(S.sh);
+  sa.squeue = (const u64 *)dp(S.queue);
+  sa.sack = (u64 *)dp(S.ack);
+  sa.scancel = (const u64 *)dp(S.cancel);
+  sa.slots = (ServeSlot *)dp(S.slots);
+  sa.nslots = S.nslots;
+  sa.qcap = S.qcap;
+  sa.sv = (ServeDev *)S.sv.p;
+  sa.sw = (u64 *)S.sw.p;
+  sa.dreq = (ServeReq *)S.dreq.p;
+  sa.dcancel = (u64 *)S.dcancel.p;
+  sa.watchdog_ticks = (u64)wd_ms * (u64)rate_khz;
+  sa.refresh_ticks = (u64)rate_khz / 100;  // 10 us
+  S.free_slots.clear();
+  for (int s = S.nslots - 1; s >= 0; s--) S.free_slots.push_back(s);
+  S.slot_ticket.assign(S.nslots, 0);
+  S.slot_cost.assign(S.nslots, 0);
+  S.on = true;
+  zr::ctx().serving = true;
+  hipLaunchKernelGGL(k_search, dim3(grid), dim3(64), 0, st, sa);
+  SV_CHECK(hipGetLastError());
+#undef SV_CHECK
+  return ZETA_OK;
+}
+
+extern "C" int zeta_serve_cost(int workers, uint64_t *bytes) {
+  int r = sv_require();
+  if (r) return r;
+  if (!bytes) return zr::fail(ZETA_EINVAL, "zeta_serve_cost: null out pointer");
+  if (workers <= 0) workers = g_sv.workers;
+  if (workers > g_sv.workers) workers = g_sv.workers;
+  *bytes = sv_cost(workers);
+  return ZETA_OK;
+}
+
+extern "C" int zeta_serve_submit(const zeta_serve_request *req, uint64_t *ticket) {
+  int r = sv_require();
+  if (r) return r;
+  ServeState &S = g_sv;
+  sv_heartbeat();
+  if (!req || !ticket) return zr::fail(ZETA_EINVAL, "zeta_serve_submit: null argument");
+  if (req->depth < 1 || req->depth > MAXPLY - 1)
+    return zr::fail(ZETA_EINVAL, "zeta_serve_submit: depth %d not in 1..%d", req->depth, MAXPLY - 1);
+  if ((r = sv_check_alive())) return r;
+  int workers = req->workers <= 0 ? S.workers : req->workers;
+  if (workers > S.workers) workers = S.workers;
+  const u64 cost = sv_cost(workers);
+  if (S.budget && cost > S.budget)
+    return zr::fail(ZETA_ENOMEM, "zeta_serve_submit: request needs %llu bytes, budget is %llu",
+                    (unsigned long long)cost, (unsigned long long)S.budget);
+  if (S.free_slots.empty()) return zr::fail(ZETA_EBUSY, "zeta_serve_submit: all %d slots in use", S.nslots);
+  if (S.budget && S.in_use + cost > S.budget)
+    return zr::fail(ZETA_EBUSY, "zeta_serve_submit: memory budget in use (%llu of %llu bytes)",
+                    (unsigned long long)S.in_use, (unsigned long long)S.budget);
+  const u64 t = S.next_ticket;
+  const int qi = (int)(t % (u64)S.qcap);
+  // the queue entry's previous ticket must have been claimed
+  if (t >= (u64)S.qcap && __atomic_load_n(&S.ack[qi], __ATOMIC_ACQUIRE) != t - (u64)S.qcap + 1)
+    return zr::fail(ZETA_EBUSY, "zeta_serve_submit: claim queue full");
+  const int s = S.free_slots.back();
+  S.free_slots.pop_back();
+  ServeSlot &sl = S.slots[s];
+  memcpy(sl.req.pos, &req->pos, sizeof(zeta_pos));
+  sl.req.depth = req->depth;
+  sl.req.flags = req->flags;
+  sl.req.workers = workers;
+  sl.req.pad = 0;
+  __atomic_store_n(&sl.state, 0u, __ATOMIC_RELAXED);
+  __atomic_store_n(&S.cancel[s], 0ULL, __ATOMIC_RELAXED);
+  __atomic_store_n(&S.queue[qi], (u64)s, __ATOMIC_RELAXED);
+  S.next_ticket = t + 1;
+  __atomic_store_n(&S.sh->submitted, t + 1, __ATOMIC_RELEASE);
+  S.slot_ticket[s] = t + 1;
+  S.slot_cost[s] = cost;
+  S.in_use += cost;
+  *ticket = t;
+  return ZETA_OK;
+}
+
+extern "C" int zeta_serve_poll(uint64_t ticket, zeta_result *res) {
+  int r = sv_require();
+  if (r) return r;
+  sv_heartbeat();
+  if (!res) return zr::fail(ZETA_EINVAL, "zeta_serve_poll: null result");
+  for (int s = 0; s < g_sv.nslots; s++) {
+    if (g_sv.slot_ticket[s] != ticket + 1) continue;
+    if (!__atomic_load_n(&g_sv.slots[s].state, __ATOMIC_ACQUIRE)) {
+      if ((r = sv_check_alive())) return r;
+      return ZETA_PENDING;
+    }
+    sv_take(s, res);
+    return ZETA_OK;
+  }
+  return zr::fail(ZETA_EINVAL, "zeta_serve_poll: ticket %llu is not in flight", (unsigned long long)ticket);
+}
+
+extern "C" int zeta_serve_reap(zeta_serve_done *out, int max, int *n) {
+  int r = sv_require();
+  if (r) return r;
+  sv_heartbeat();
+  if (!out || !n || max < 0) return zr::fail(ZETA_EINVAL, "zeta_serve_reap: bad arguments");
+  int k = 0;
+  for (int s = 0; s < g_sv.nslots && k < max; s++) {
+    if (!g_sv.slot_ticket[s] || !__atomic_load_n(&g_sv.slots[s].state, __ATOMIC_ACQUIRE)) continue;
+    out[k].ticket = g_sv.slot_ticket[s] - 1;
+    sv_take(s, &out[k].res);
+    k++;
+  }
+  *n = k;
+  if (k == 0 && (r = sv_check_alive())) return r;
+  return ZETA_OK;
+}
+
+extern "C" int zeta_serve_cancel(uint64_t ticket) {
+  int r = sv_require();
+  if (r) return r;
+  sv_heartbeat();
+  for (int s = 0; s < g_sv.nslots; s++) {
+    if (g_sv.slot_ticket[s] != ticket + 1) continue;
+    __atomic_store_n(&g_sv.cancel[s], ticket + 1, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&g_sv.sh->cancel_gen, 1u, __ATOMIC_RELEASE);
+    return ZETA_OK;
+  }
+  return zr::fail(ZETA_EINVAL, "zeta_serve_cancel: ticket %llu is not in flight", (unsigned long long)ticket);
+}
+
+extern "C" int zeta_serve_stop(uint64_t *total_nodes) {
+  int r = sv_require();
+  if (r) return r;
+  ServeState &S = g_sv;
+  __atomic_store_n(&S.sh->shutdown, 1u, __ATOMIC_RELEASE);
+  sv_heartbeat();
+  hipError_t e = hipStreamSynchronize(zr::ctx().stream);
+  // requests finished but not collected count towards the total
+  for (int s = 0; s < S.nslots; s++)
+    if (S.slot_ticket[s] && __atomic_load_n(&S.slots[s].state, __ATOMIC_ACQUIRE)) S.total_nodes += S.slots[s].nodes;
+  u64 ctr[C_NCTR] = {0};
+  u32 err = 0;
+  if (e == hipSuccess) e = hipMemcpy(ctr, S.ctr.p, sizeof(ctr), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&err, S.err.p, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) memcpy(g_ss.last_ctr, ctr, sizeof(ctr));
+  if (total_nodes) *total_nodes = S.total_nodes;
+  sv_free_all();
+  if (e != hipSuccess) return zr::hip_fail(e, "zeta_serve_stop", __FILE__, __LINE__);
   return ZETA_OK;
 }
