@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--no-search", action="store_true", help="skip the fixed-depth search leg")
     ap.add_argument("--search-depth", type=int, default=6)
     ap.add_argument("--search-workers", type=int, default=64)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
+                    help="also run the library's streaming copy kernel once over BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (positions shard across GPUs with a "
@@ -111,6 +113,15 @@ def main():
                 "kernel": "perft_leaf", "avg_launch_ms": round(leaf[1] / leaf[0], 4),
                 "items_per_launch": int(per_launch_items), "ops64_per_item": ops[key]}
 
+    calib = None
+    if args.calib_copy:
+        nb = (args.calib_copy + 15) // 16 * 16
+        a, b = zeta_amd.DeviceBuffer(nb), zeta_amd.DeviceBuffer(nb)
+        zeta_amd.copy_device(b.ptr, a.ptr, nb)
+        a.free()
+        b.free()
+        calib = {"copy_bytes": nb}
+
     # config 1: startpos perft(5), one root (the reference's own perft command)
     start = np.zeros(1, dtype=zeta_amd.POS_DTYPE)
     start["b"][0] = [0xFFFF000000000000, 0x91FF00000000FF91, 0x5A0000000000005A, 0xAD000000000000AD,
@@ -155,6 +166,8 @@ def main():
            "config1": {"workload": "startpos perft(5), one root", "leaves": c1, "ms": round(c1_ms, 3),
                        "leaves_per_s": c1 / (c1_ms / 1e3)},
            "search": search}
+    if calib:
+        out["calibration"] = calib
     print(json.dumps(out))
 
 

@@ -100,7 +100,7 @@ typedef struct {
   int use_mlp;        /* learned move ordering at interior nodes (needs zeta_mlp_load) */
   int mlp_min_depth;  /* nodes with depth >= this use the learned ordering */
   int log_records;    /* record cutoff examples for training (zeta_records_*) */
-  int reserved;
+  int log_shift;      /* log the cut nodes whose hash has its low log_shift bits (of bits 40..) zero */
 } zeta_search_opts;
 
 typedef struct {
@@ -209,6 +209,11 @@ int zeta_dev_alloc(void **dptr, size_t bytes);
 int zeta_dev_free(void *dptr);
 int zeta_memcpy_htod(void *dst, const void *src, size_t bytes);
 int zeta_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by the library's own streaming kernel (16-byte
+ * aligned pointers, size a multiple of 16).  Besides copying, it is the
+ * known-size kernel that calibrates HBM byte counters (FETCH_SIZE,
+ * WRITE_SIZE): it reads and writes exactly `bytes` each. */
+int zeta_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int zeta_synchronize(void);
 
 /* ------------------------------------------------------------------ stats */

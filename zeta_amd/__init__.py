@@ -44,7 +44,7 @@ class _Stats(ctypes.Structure):
 class SearchOpts(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_int), ("workers", ctypes.c_int), ("tt_log2", ctypes.c_int),
                 ("clear_tt", ctypes.c_int), ("max_nodes", ctypes.c_uint64), ("use_mlp", ctypes.c_int),
-                ("mlp_min_depth", ctypes.c_int), ("log_records", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("mlp_min_depth", ctypes.c_int), ("log_records", ctypes.c_int), ("log_shift", ctypes.c_int)]
 
 
 RESULT_DTYPE = np.dtype([("score", "<i4"), ("bestmove", "<u4"), ("nodes", "<u8"), ("depth", "<i4"),
@@ -112,6 +112,7 @@ def lib():
         L.zeta_dev_free.argtypes = [vp]
         L.zeta_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
         L.zeta_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
+        L.zeta_memcpy_dtod.argtypes = [vp, vp, ctypes.c_size_t]
         L.zeta_serve_start.argtypes = [ctypes.POINTER(ServeOpts)]
         L.zeta_serve_cost.argtypes = [i32, vp]
         L.zeta_serve_submit.argtypes = [ctypes.POINTER(ServeRequest), vp]
@@ -122,7 +123,7 @@ def lib():
         for f in ("zeta_init", "zeta_shutdown", "zeta_last_error", "zeta_device_info",
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
                   "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
-                  "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_synchronize",
+                  "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_memcpy_dtod", "zeta_synchronize",
                   "zeta_search_batch", "zeta_search_counters", "zeta_mlp_init", "zeta_mlp_set",
                   "zeta_mlp_get", "zeta_mlp_grad", "zeta_mlp_train", "zeta_records_reserve",
                   "zeta_records_count", "zeta_records_clear", "zeta_records_get", "zeta_records_put",
@@ -198,7 +199,7 @@ def perft_device(d_pos, n, depth, d_counts, nobulk=False):
 
 
 def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_mlp=False,
-           mlp_min_depth=3, log_records=False):
+           mlp_min_depth=3, log_records=False, log_shift=0):
     """Fixed-depth alpha-beta + quiescence of every position (iterative
     deepening 1..depth).  Returns a structured array (score, bestmove, nodes,
     depth, status) -- the mirror of rootsearch's PV[0]/PV[1]/node count."""
@@ -206,7 +207,7 @@ def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_ml
     pos = as_positions(pos)
     out = np.zeros(len(pos), dtype=RESULT_DTYPE)
     o = SearchOpts(int(depth), int(workers), int(tt_log2), int(bool(clear_tt)), int(max_nodes),
-                   int(bool(use_mlp)), int(mlp_min_depth), int(bool(log_records)), 0)
+                   int(bool(use_mlp)), int(mlp_min_depth), int(bool(log_records)), int(log_shift))
     _check(lib().zeta_search_batch(_ptr(pos), len(pos), ctypes.byref(o), _ptr(out)), "zeta_search_batch")
     return out
 
@@ -254,6 +255,12 @@ class DeviceBuffer:
         if self.ptr:
             _check(lib().zeta_dev_free(ctypes.c_void_p(self.ptr)), "zeta_dev_free")
             self.ptr = None
+
+
+def copy_device(dst_ptr, src_ptr, nbytes):
+    """Device-to-device copy by the library's streaming kernel (16-byte multiples)."""
+    _check(lib().zeta_memcpy_dtod(ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr), int(nbytes)),
+           "zeta_memcpy_dtod")
 
 
 def synchronize():

@@ -26,6 +26,7 @@ enum KernelId {
   K_MLP_BWD,
   K_MLP_ADAM,
   K_MLP_SCORE,
+  K_COPY,
   K_COUNT_IDS
 };
 

@@ -69,7 +69,8 @@ void release(DevBuf &b) {
 static const char *kname(int id) {
   static const char *names[K_COUNT_IDS] = {"perft_roots", "perft_count", "perft_scan", "perft_expand",
                                            "perft_apply", "perft_leaf",  "playout",    "search",
-                                           "mlp_fwd",     "mlp_bwd",     "mlp_adam",   "mlp_score"};
+                                           "mlp_fwd",     "mlp_bwd",     "mlp_adam",   "mlp_score",
+                                           "copy"};
   return names[id];
 }
 
@@ -255,6 +256,33 @@ extern "C" int zeta_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   if (bytes && (!dst || !src)) return fail(ZETA_EINVAL, "zeta_memcpy_dtoh: null pointer");
   ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx().stream));
   ZCHECK(hipStreamSynchronize(ctx().stream));
+  return ZETA_OK;
+}
+
+// streaming copy, 16 B per lane per iteration (nontemporal: the data is not reused)
+typedef unsigned int cu32x4 __attribute__((ext_vector_type(4)));
+__global__ void k_copy16(const cu32x4 *__restrict__ src, cu32x4 *__restrict__ dst, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+    const cu32x4 v = __builtin_nontemporal_load(src + i);
+    __builtin_nontemporal_store(v, dst + i);
+  }
+}
+
+extern "C" int zeta_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  int r = require_up();
+  if (r) return r;
+  if (!bytes) return ZETA_OK;
+  if (!dst || !src) return fail(ZETA_EINVAL, "zeta_memcpy_dtod: null pointer");
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return fail(ZETA_EINVAL, "zeta_memcpy_dtod: pointers and size must be multiples of 16 bytes");
+  const size_t n16 = bytes / 16;
+  size_t blocks = (n16 + 255) / 256;
+  const size_t cap = (size_t)ctx().cu_count * 8;
+  if (blocks > cap) blocks = cap;
+  ZLAUNCH(K_COPY, bytes, k_copy16, (unsigned)blocks, 256, 0, (const cu32x4 *)src, (cu32x4 *)dst, n16);
+  ZCHECK(hipStreamSynchronize(ctx().stream));
+  time_collect();
   return ZETA_OK;
 }
 

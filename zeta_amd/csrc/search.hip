@@ -129,6 +129,7 @@ struct SearchArgs {
   u64 *recs;
   u64 *rec_count;
   u64 rec_cap;
+  u32 rec_mask;  // a cut node is logged when (hash >> 40) & rec_mask == 0
   // serving mode (slots != nullptr)
   ServeHost *sh;
   const u64 *squeue;  // host: slot of ticket t at [t % qcap]
@@ -439,16 +440,27 @@ __device__ void mlp_score_moves(const SearchArgs &a, const Pos &p, int mbeg, int
 }
 
 // training pairs at a cut node: the cutoff move against every move searched
-// before it at this node (scores marked USED), wave-compacted
+// before it (scores marked USED, the hard negatives) and against a
+// hash-selected sample of about four of the moves never searched, so nodes
+// whose first move cut contribute too; wave-compacted
 __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u32 *mmove, const int16_t *mscore,
                           u32 &nrec) {
+  const u32 thresh = n.mcnt > 4 ? (u32)(4 * 256 / n.mcnt) : 256u;
   for (int c0 = 0; c0 < n.mcnt; c0 += 64) {
     const int k = c0 + lane;
     bool neg = false;
     u32 m = 0;
     if (k < n.mcnt) {
       m = mmove[n.mbeg + k];
-      neg = mscore[n.mbeg + k] == (int16_t)USED && m != n.move;
+      if (m != n.move) {
+        if (mscore[n.mbeg + k] == (int16_t)USED) {
+          neg = true;
+        } else {
+          u64 z = n.hash ^ ((u64)m * 0x9E3779B97F4A7C15ULL);
+          z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ULL;
+          neg = (u32)((z >> 40) & 0xFF) < thresh;
+        }
+      }
     }
     const u64 mask = __ballot(neg);
     if (!mask) continue;
@@ -979,7 +991,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           }
         }
         if (n.alpha >= n.beta) {
-          if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 2) log_pairs(a, n, lane, mmove, mscore, c_rec);
+          if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 1 && n.mcnt >= 2 &&
+              ((u32)(n.hash >> 40) & a.rec_mask) == 0)
+            log_pairs(a, n, lane, mmove, mscore, c_rec);
           if (((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0 && killers[sd][0] != n.move) {
             killers[sd][1] = killers[sd][0];
             killers[sd][0] = n.move;
@@ -1160,6 +1174,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.recs = nullptr;
   sa.rec_count = nullptr;
   sa.rec_cap = 0;
+  sa.rec_mask = (opts->log_shift > 0 && opts->log_shift < 24) ? ((1u << opts->log_shift) - 1u) : 0u;
   sa.sh = nullptr;
   sa.squeue = nullptr;
   sa.sack = nullptr;

@@ -1,0 +1,738 @@
+// zeta_engine.cpp -- CECP (xboard) front end of the MI355X build.
+//
+// The command set, the output strings and the config.txt keys follow the
+// reference's front end (src/zeta.c:525-709, src/xboard.c:44-1499,
+// src/io.c:335-582, src/search.c:35-271, src/test.c:43-140); the device work
+// goes through the reference's six-function boundary (include/zeta_clrun.h),
+// here backed by libzetahip.so.  Position handling (FEN, move text, make /
+// unmake) uses the same chess core as the kernels (zeta_amd/csrc/chess.h,
+// host build).
+#include <getopt.h>
+#include <inttypes.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/zeta_clrun.h"
+#include "../../include/zeta_hip.h"
+#include "../csrc/chess.h"
+
+using zc::Legal;
+using zc::Pos;
+
+// ------------------------------------------------------- process globals
+// (src/zeta.c:56-110), read and written by the clrun boundary
+uint64_t threadsX = 1;
+uint64_t threadsY = 1;
+const uint64_t threadsZ = 64;
+uint64_t totalWorkUnits = 1;
+uint64_t tt1_memory = 0;
+uint64_t tt2_memory = 0;
+int32_t opencl_device_id = 0;
+int32_t opencl_platform_id = 0;
+int32_t PLY = 0;
+uint64_t *GLOBAL_BOARD = nullptr;
+uint64_t *GLOBAL_HASHHISTORY = nullptr;
+uint64_t *COUNTERS = nullptr;
+uint32_t *PV = nullptr;
+
+namespace {
+
+constexpr const char *VERSION = "099m-mi355x";
+constexpr int MAXPLY = 64;         // src/types.h:76
+constexpr int MAXGAMEPLY = 1024;   // src/types.h:77
+constexpr int ESTEBF = 3;          // src/types.h:81
+constexpr int32_t INF = 32000;     // src/types.h:87
+constexpr int32_t MATESCORE = 30000;
+constexpr const char *STARTFEN = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1";
+constexpr const char *LOGFILE = "zeta.log";
+
+// engine state (src/zeta.c:67-110)
+uint64_t BOARD[7];
+bool STM = false;
+int32_t SD = MAXPLY;
+uint64_t MaxNodes = 1;
+int64_t nodes_per_second = 0;
+double MaxTime = 1e12, TimeBase = 0, TimeInc = 0, TimeLeft = 0;
+int32_t MovesLeft = 1, MaxMoves = 1, timemode = 0;
+uint64_t ABNODECOUNT = 0, TTHITS = 0, TTSCOREHITS = 0, IIDHITS = 0;
+bool xboard_mode = false, xboard_force = false, xboard_post = false, xboard_debug = false;
+int xboard_protover = 0;
+FILE *LogFile = nullptr;
+const char *configfile = "config.txt";
+
+// game history: the position before each ply, for undo / remove
+struct Undo {
+  uint64_t b[7];
+  bool stm;
+};
+std::vector<Undo> history;
+std::vector<uint64_t> HashHistory(MAXGAMEPLY + 1, 0);
+
+double now_ms() {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
+}
+
+void logf_(const char *fmt, ...) {
+  if (!LogFile) return;
+  time_t t = time(nullptr);
+  char d[64];
+  strftime(d, sizeof d, "[%Y-%m-%d %H:%M:%S] ", localtime(&t));
+  fputs(d, LogFile);
+  va_list ap;
+  va_start(ap, fmt);
+  vfprintf(LogFile, fmt, ap);
+  va_end(ap);
+  fflush(LogFile);
+}
+
+// print to stdout and to the log
+void out(const char *fmt, ...) {
+  char buf[4096];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  fputs(buf, stdout);
+  fflush(stdout);
+  logf_("%s", buf);
+}
+
+[[noreturn]] void quitengine(int code) {
+  (void)cl_release_device();
+  free(GLOBAL_BOARD);
+  free(GLOBAL_HASHHISTORY);
+  free(COUNTERS);
+  free(PV);
+  if (LogFile) fclose(LogFile);
+  exit(code);
+}
+
+// ------------------------------------------------------------ positions
+Pos to_pos(const uint64_t *b, bool stm) {
+  Pos p;
+  p.b0 = b[0];
+  p.b1 = b[1];
+  p.b2 = b[2];
+  p.b3 = b[3];
+  p.cr = b[4];
+  p.stm = stm ? 1u : 0u;
+  return p;
+}
+
+std::vector<uint32_t> legal_moves(const uint64_t *b, bool stm) {
+  const Pos p = to_pos(b, stm);
+  Legal L;
+  zc::analyse(p, L);
+  std::vector<uint32_t> mv;
+  uint64_t pcs = L.own;
+  while (pcs) {
+    const int f = zc::lsb(pcs);
+    pcs &= pcs - 1;
+    uint64_t t = zc::piece_targets(p, L, f, false);
+    while (t) {
+      const int to = zc::lsb(t);
+      t &= t - 1;
+      mv.push_back(zc::make_code(p, f, to));
+    }
+  }
+  return mv;
+}
+
+bool in_check(const uint64_t *b, bool stm) { return zc::in_check(to_pos(b, stm)); }
+
+// host domove (src/bitboard.c:583-693): planes, moved flags with the castle
+// rook, e.p. flag, half-move clock, hash from scratch
+void domove(uint64_t *b, bool stm, uint32_t m) {
+  if (!(m & 0x3FFFFFFF)) return;
+  const Pos c = zc::apply(to_pos(b, stm), m, true);
+  b[0] = c.b0;
+  b[1] = c.b1;
+  b[2] = c.b2;
+  b[3] = c.b3;
+  b[4] = c.cr;
+  b[6] = zc::next_hmc((uint32_t)b[6], m);
+  b[5] = zc::hash_full(c);
+}
+
+// FEN -> quad-bitboard with the reference's rules (src/io.c:335-532): moved
+// flags start all set, castle letters clear rook and king squares, the FEN
+// e.p. square is cleared on rank 3/6 where the move generator never looks.
+bool setboard(uint64_t *b, bool &stm, const char *fen) {
+  char position[1024] = {0}, cstm[16] = {0}, castle[16] = {0}, cep[16] = {0};
+  unsigned long long hmc = 0, fullmove = 1;
+  if (sscanf(fen, "%1023s %15s %15s %15s %llu %llu", position, cstm, castle, cep, &hmc, &fullmove) < 1)
+    return false;
+  uint64_t nb[7] = {0, 0, 0, 0, ~0ULL, 0, 0};
+  int rank = 7, file = 0;
+  for (const char *q = position; *q; q++) {
+    const char ch = *q;
+    if (ch == '/') {
+      rank--;
+      file = 0;
+      if (rank < 0) return false;
+      continue;
+    }
+    if (ch >= '1' && ch <= '8') {
+      file += ch - '0';
+      continue;
+    }
+    const char *types = "pnkbrq";  // piece type 1..6 (src/types.h:94-100)
+    const char lower = (char)(ch | 0x20);
+    const char *tp = strchr(types, lower);
+    if (!tp || !*tp || file > 7) return false;
+    const uint32_t type = (uint32_t)(tp - types) + 1;
+    const uint32_t color = (ch == lower) ? 1u : 0u;
+    const int sq = rank * 8 + file;
+    const uint64_t m = 1ULL << sq;
+    if (color) nb[0] |= m;
+    if (type & 1) nb[1] |= m;
+    if (type & 2) nb[2] |= m;
+    if (type & 4) nb[3] |= m;
+    file++;
+  }
+  if (rank != 0) return false;
+  uint64_t cr = 0;
+  if (castle[0] != '-')
+    for (const char *q = castle; *q; q++) {
+      if (*q == 'Q') cr |= zc::SMCRWHITEQ;
+      if (*q == 'K') cr |= zc::SMCRWHITEK;
+      if (*q == 'q') cr |= zc::SMCRBLACKQ;
+      if (*q == 'k') cr |= zc::SMCRBLACKK;
+    }
+  nb[4] ^= cr;
+  if (cep[0] >= 'a' && cep[0] <= 'h' && cep[1] >= '1' && cep[1] <= '8') {
+    const int sq = (cep[1] - '1') * 8 + (cep[0] - 'a');
+    if (sq > 0) nb[4] &= ~(1ULL << sq);
+  }
+  nb[6] = hmc;
+  const bool nstm = (cstm[0] == 'b' || cstm[0] == 'B');
+  // one king per side (src/bitboard.c:448-459)
+  const uint64_t kings = nb[1] & nb[2] & ~nb[3];
+  const uint64_t occ = nb[1] | nb[2] | nb[3];
+  if (__builtin_popcountll(nb[0] & kings) != 1 || __builtin_popcountll((nb[0] ^ occ) & kings) != 1) return false;
+  nb[5] = zc::hash_full(to_pos(nb, nstm));
+  memcpy(b, nb, sizeof nb);
+  stm = nstm;
+  return true;
+}
+
+std::string move2can(uint32_t m) {
+  if (!(m & 0x3FFFFFFF)) return "0000";
+  const int f = (int)zc::mv_from(m), t = (int)zc::mv_to(m);
+  std::string s;
+  s += (char)('a' + (f & 7));
+  s += (char)('1' + (f >> 3));
+  s += (char)('a' + (t & 7));
+  s += (char)('1' + (t >> 3));
+  const uint32_t pf = (zc::mv_pfrom(m) >> 1) & 7, pt = (zc::mv_pto(m) >> 1) & 7;
+  if (pf == 1 && pt != 1) s += "?pnkbrq"[pt];
+  return s;
+}
+
+// text -> legal move (the promotion piece is always a queen, Q1); 0 if illegal
+uint32_t can2move(const char *s, const uint64_t *b, bool stm) {
+  if (strlen(s) < 4) return 0;
+  const int f = (s[0] - 'a') + 8 * (s[1] - '1'), t = (s[2] - 'a') + 8 * (s[3] - '1');
+  for (uint32_t m : legal_moves(b, stm))
+    if ((int)zc::mv_from(m) == f && (int)zc::mv_to(m) == t) return m;
+  return 0;
+}
+
+void printboard(const uint64_t *b) {
+  static const char *chars = ".PNKBRQ.pnkbrq";
+  std::string s = "###ABCDEFGH###\n";
+  for (int r = 7; r >= 0; r--) {
+    s += "#" + std::to_string(r + 1) + " ";
+    for (int f = 0; f < 8; f++) {
+      const int sq = r * 8 + f;
+      const uint32_t pc = zc::piece_at(to_pos(b, false), sq);
+      const uint32_t type = (pc >> 1) & 7, col = pc & 1;
+      s += type ? chars[type + (col ? 7 : 0)] : '.';
+    }
+    s += "\n";
+  }
+  out("%s", s.c_str());
+}
+
+// --------------------------------------------------------- host drivers
+void fill_history() {
+  for (uint64_t w = 0; w < totalWorkUnits; w++)
+    for (int i = 0; i <= PLY && i < MAXGAMEPLY; i++) GLOBAL_HASHHISTORY[w * MAXGAMEPLY + i] = HashHistory[i];
+}
+
+// src/search.c:225-271
+void perft(const uint64_t *board, bool stm, int32_t depth) {
+  ABNODECOUNT = 0;
+  memcpy(GLOBAL_BOARD, board, 7 * sizeof(uint64_t));
+  fill_history();
+  if (!cl_write_objects() || !cl_run_perft(stm, depth) || !cl_read_memory()) quitengine(EXIT_FAILURE);
+  for (uint64_t i = 0; i < totalWorkUnits; i++) ABNODECOUNT += COUNTERS[i * 64 + 1];
+}
+
+bool is_mate(int32_t v) { return (v > MATESCORE && v < INF) || (v < -MATESCORE && v > -INF); }
+
+// src/search.c:35-223: iterative deepening over single-depth device calls
+uint32_t rootsearch(const uint64_t *board, bool stm, int32_t depth) {
+  uint32_t bestmove = 0;
+  int32_t bestscore = 0;
+  ABNODECOUNT = TTHITS = TTSCOREHITS = IIDHITS = 0;
+  const double start = now_ms();
+  memcpy(GLOBAL_BOARD, board, 7 * sizeof(uint64_t));
+  fill_history();
+  if (!xboard_mode) out("depth score time nodes pv \n");
+  int32_t idf = 1;
+  double elapsed = 0;
+  do {
+    if (!cl_write_objects() || !cl_run_alphabeta(stm, idf, MaxNodes / totalWorkUnits) || !cl_read_memory())
+      quitengine(EXIT_FAILURE);
+    for (uint64_t i = 0; i < totalWorkUnits; i++) {
+      ABNODECOUNT += COUNTERS[i * 64 + 1];
+      TTHITS += COUNTERS[i * 64 + 3];
+      TTSCOREHITS += COUNTERS[i * 64 + 4];
+      IIDHITS += COUNTERS[i * 64 + 5];
+    }
+    elapsed = (now_ms() - start + 1) / 1000;
+    const uint32_t pvmove = PV[1] & 0x3FFFFFFF;
+    if (idf == 1 && pvmove) bestmove = PV[1];
+    if (COUNTERS[1] < MaxNodes / totalWorkUnits) {
+      if (pvmove) bestmove = PV[1];
+      const int32_t s = (int32_t)PV[0];
+      bestscore = (s == INF || s == -INF) ? 0 : s;
+      int32_t xs = bestscore;
+      if (bestscore <= -MATESCORE) xs = -100000 - (INF + bestscore);
+      if (bestscore >= MATESCORE) xs = 100000 - (-INF + bestscore);
+      if ((xboard_post || !xboard_mode) && pvmove) {
+        out("%i %i %i %" PRIu64 " %s \n", idf, xs, (int32_t)(elapsed * 100), ABNODECOUNT,
+            move2can(PV[1]).c_str());
+      } else {
+        break;
+      }
+    }
+    if (is_mate(bestscore) && bestscore >= MATESCORE && idf >= INF - bestscore) break;
+    if (is_mate(bestscore) && bestscore <= -MATESCORE && idf >= INF + bestscore) break;
+    idf++;
+  } while (idf <= depth && elapsed * ESTEBF < MaxTime && ABNODECOUNT * ESTEBF <= MaxNodes && ABNODECOUNT > 1 &&
+           idf < MAXPLY);
+  if (!xboard_mode || xboard_debug)
+    out("#%" PRIu64 " searched nodes in %lf seconds, with %" PRIu64 " ttmovehits, and %" PRIu64
+        " ttscorehits, %" PRIu64 " iidhits, ebf: %lf, nps: %" PRIu64 " \n",
+        ABNODECOUNT, elapsed, TTHITS, TTSCOREHITS, IIDHITS, pow((double)ABNODECOUNT, 1.0 / idf),
+        (uint64_t)((double)ABNODECOUNT / elapsed));
+  // nps estimate for time control (src/search.c:214-220)
+  const int64_t nps = (int64_t)((double)ABNODECOUNT / elapsed);
+  nodes_per_second += (nps > nodes_per_second) ? (int64_t)((nps - nodes_per_second) * 0.66)
+                                               : (int64_t)((nps - nodes_per_second) * 0.33);
+  return bestmove;
+}
+
+// ----------------------------------------------------------- game flow
+void push_move(uint32_t m) {
+  history.push_back({{BOARD[0], BOARD[1], BOARD[2], BOARD[3], BOARD[4], BOARD[5], BOARD[6]}, STM});
+  domove(BOARD, STM, m);
+  PLY++;
+  STM = !STM;
+  if (PLY <= MAXGAMEPLY) HashHistory[PLY] = BOARD[5];
+}
+
+void pop_move() {
+  if (history.empty()) return;
+  const Undo &u = history.back();
+  memcpy(BOARD, u.b, sizeof BOARD);
+  STM = u.stm;
+  history.pop_back();
+  PLY--;
+}
+
+void new_game(const char *fen) {
+  bool stm = false;
+  uint64_t b[7];
+  if (!setboard(b, stm, fen)) {
+    out("Error (in setting chess psotition via fen string): setboard\n");
+    out("tellusererror (Error in setting chess psotition via fen string): setboard\n");
+    return;
+  }
+  memcpy(BOARD, b, sizeof BOARD);
+  STM = stm;
+  PLY = 0;
+  history.clear();
+  HashHistory.assign(MAXGAMEPLY + 1, 0);
+  HashHistory[0] = BOARD[5];
+}
+
+void time_control(double elapsed_ms) {
+  TimeLeft -= elapsed_ms;
+  if (timemode == 1) MovesLeft = (MaxMoves - (((PLY + 1) / 2) % MaxMoves)) + 1;
+  if (timemode == 0) TimeLeft = TimeBase;
+  if (timemode == 2) TimeLeft += TimeInc;
+  MaxTime = TimeLeft / MovesLeft + TimeInc;
+  if (timemode == 1 && MovesLeft == 2) TimeLeft += TimeBase;
+  MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+}
+
+// think and play a move (src/xboard.c:217-376)
+void think() {
+  const bool kic = in_check(BOARD, STM);
+  const double start = now_ms();
+  HashHistory[PLY] = BOARD[5];
+  if (PLY >= MAXGAMEPLY) {
+    out(STM ? "result 1-0 { resign - max game ply reached }\n" : "result 0-1 { resign - max game ply reached}\n");
+    return;
+  }
+  const uint32_t move = rootsearch(BOARD, STM, SD);
+  const bool none = !(move & 0x3FFFFFFF);
+  if (ABNODECOUNT == 0 && !none) {
+    out(STM ? "result 1-0 { resign - internal error }\n" : "result 0-1 { resign - internal error}\n");
+  } else if (legal_moves(BOARD, STM).empty() && kic) {
+    out(STM ? "result 1-0 { checkmate }\n" : "result 0-1 { checkmate }\n");
+  } else if (legal_moves(BOARD, STM).empty()) {
+    out("result 1/2-1/2 { stalemate }\n");
+  } else {
+    out("move %s\n", move2can(move).c_str());
+    push_move(move);
+    if (!xboard_mode || xboard_debug) printboard(BOARD);
+    time_control(now_ms() - start);
+  }
+}
+
+// ---------------------------------------------------------------- tests
+// the reference's known-answer perft positions (src/test.c:43-73)
+struct Kat {
+  const char *fen;
+  int depth;
+  uint64_t nodes;
+};
+const Kat KATS[] = {
+    {"rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq -", 1, 20},
+    {"rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq -", 2, 400},
+    {"rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq -", 3, 8902},
+    {"rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq -", 4, 197281},
+    {"8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - ", 1, 14},
+    {"8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - ", 2, 191},
+    {"8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - ", 3, 2812},
+    {"8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - ", 4, 43238},
+    {"r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - ", 1, 48},
+    {"r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - ", 2, 2039},
+    {"r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - ", 3, 97862},
+    {"r4rk1/1pp1qppp/p1np1n2/2b1p1B1/2B1P1b1/P1NP1N2/1PP1QPPP/R4RK1 w - -", 1, 46},
+    {"r4rk1/1pp1qppp/p1np1n2/2b1p1B1/2B1P1b1/P1NP1N2/1PP1QPPP/R4RK1 w - -", 2, 2079},
+    {"r4rk1/1pp1qppp/p1np1n2/2b1p1B1/2B1P1b1/P1NP1N2/1PP1QPPP/R4RK1 w - -", 3, 89890},
+};
+
+// src/test.c:43-140
+bool selftest() {
+  const uint64_t todo = sizeof KATS / sizeof KATS[0];
+  uint64_t passed = 0;
+  uint64_t b[7];
+  bool stm = false;
+  for (uint64_t done = 0; done < todo; done++) {
+    out("#> doing perft depth: %d for position %" PRIu64 " of %" PRIu64 "\n", KATS[done].depth, done + 1, todo);
+    if (!setboard(b, stm, KATS[done].fen)) {
+      out("# Error (in setting fen position): setboard\n");
+      continue;
+    }
+    printboard(b);
+    const double start = now_ms();
+    perft(b, stm, KATS[done].depth);
+    const double el = (now_ms() - start) / 1000;
+    if (ABNODECOUNT == KATS[done].nodes) {
+      passed++;
+      out("#> OK, Nodecount Correct, %" PRIu64 " nodes in %lf seconds with %" PRIu64 " nps.\n", ABNODECOUNT, el,
+          (uint64_t)(ABNODECOUNT / (el > 0 ? el : 1e-9)));
+    } else {
+      out("#> Error, Nodecount NOT Correct, %" PRIu64 " computed nodes != %" PRIu64
+          " nodes for depth %d. in %lf seconds with %" PRIu64 " nps.\n",
+          ABNODECOUNT, KATS[done].nodes, KATS[done].depth, el, (uint64_t)(ABNODECOUNT / (el > 0 ? el : 1e-9)));
+    }
+  }
+  out("#> %" PRIu64 " of %" PRIu64 " tests passed\n", passed, todo);
+  return passed == todo;
+}
+
+// ---------------------------------------------------------------- config
+// src/io.c:533-582
+bool read_and_init_config(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) {
+    out("Error (%s file missing): try --guessconfigx option to create a config.txt file or --help option for "
+        "further options\n", path);
+    return false;
+  }
+  char line[1024];
+  while (fgets(line, sizeof line, f)) {
+    sscanf(line, "threadsX: %" SCNu64 ";", &threadsX);
+    sscanf(line, "threadsY: %" SCNu64 ";", &threadsY);
+    sscanf(line, "nodes_per_second: %" SCNi64 ";", &nodes_per_second);
+    sscanf(line, "tt1_memory: %" SCNu64 ";", &tt1_memory);
+    sscanf(line, "tt2_memory: %" SCNu64 ";", &tt2_memory);
+    sscanf(line, "opencl_platform_id: %d;", &opencl_platform_id);
+    sscanf(line, "opencl_device_id: %d;", &opencl_device_id);
+  }
+  fclose(f);
+  MaxNodes = (uint64_t)nodes_per_second;
+  totalWorkUnits = threadsX * threadsY;
+  if (totalWorkUnits < 1) totalWorkUnits = 1;
+  return true;
+}
+
+bool engineinits() {
+  GLOBAL_BOARD = (uint64_t *)calloc(7, sizeof(uint64_t));
+  GLOBAL_HASHHISTORY = (uint64_t *)calloc(totalWorkUnits * MAXGAMEPLY, sizeof(uint64_t));
+  COUNTERS = (uint64_t *)calloc(totalWorkUnits * threadsZ, sizeof(uint64_t));
+  PV = (uint32_t *)calloc(MAXPLY, sizeof(uint32_t));
+  return GLOBAL_BOARD && GLOBAL_HASHHISTORY && COUNTERS && PV;
+}
+
+// Autotuner (replaces --guessconfig[x], src/clconfig.c): measures search
+// throughput for a few worker counts per root on seeded positions and writes
+// config.txt with the best one, the TT size and the measured nodes per second.
+bool guess_config(bool extended) {
+  zeta_config cfg;
+  memset(&cfg, 0, sizeof cfg);
+  cfg.device = opencl_device_id;
+  cfg.tt_log2 = 24;
+  int rc = zeta_init(&cfg);
+  if (rc) {
+    char msg[512];
+    zeta_last_error(msg, sizeof msg);
+    out("Error (%s): --guessconfig\n", msg);
+    return false;
+  }
+  const int npos = extended ? 64 : 16;
+  std::vector<zeta_pos> pos(npos);
+  rc = zeta_playouts(3, npos, 20, 60, pos.data());
+  std::vector<zeta_result> res(npos);
+  int best_w = 1;
+  double best_nps = 0;
+  const int cands[] = {1, 4, 16, 64};
+  for (int w : cands) {
+    zeta_search_opts o;
+    memset(&o, 0, sizeof o);
+    o.depth = extended ? 5 : 4;
+    o.workers = w;
+    o.tt_log2 = 24;
+    o.clear_tt = 1;
+    const double t0 = now_ms();
+    rc = rc ? rc : zeta_search_batch(pos.data(), 1, &o, res.data());  // one root: the engine's case
+    const double dt = (now_ms() - t0) / 1000;
+    if (rc) break;
+    const double nps = res[0].nodes / (dt > 0 ? dt : 1e-9);
+    out("#> workers per root %d: %.0f nodes/s\n", w, nps);
+    if (nps > best_nps) {
+      best_nps = nps;
+      best_w = w;
+    }
+  }
+  zeta_shutdown();
+  if (rc) return false;
+  FILE *f = fopen(configfile, "w");
+  if (!f) return false;
+  fprintf(f, "threadsX: %d;\nthreadsY: 1;\nnodes_per_second: %" PRIu64 ";\ntt1_memory: 1024;\ntt2_memory: 0;\n"
+             "opencl_platform_id: 0;\nopencl_device_id: %d;\n",
+          best_w, (uint64_t)best_nps, opencl_device_id);
+  fclose(f);
+  out("#> config written to %s (threadsX %d, nodes_per_second %" PRIu64 ")\n", configfile, best_w,
+      (uint64_t)best_nps);
+  return true;
+}
+
+void print_help() {
+  out("Zeta %s, MI355X build. Options:\n", VERSION);
+  out(" -h, --help         Print help\n -v, --version      Print version\n");
+  out(" -l, --log          Write output/debug to file zeta.log\n -s, --selftest     Run an internal test\n");
+  out(" --guessconfig      Measure the device and write config.txt (quick)\n");
+  out(" --guessconfigx     Measure the device and write config.txt (longer)\n");
+  out(" --config FILE      Read FILE instead of ./config.txt\n");
+}
+
+// --------------------------------------------------------------- xboard
+void xboard() {
+  char line[1024], cmd[1024];
+  while (fgets(line, sizeof line, stdin)) {
+    if (line[0] == '\n') continue;
+    logf_(">> %s", line);
+    cmd[0] = 0;
+    sscanf(line, "%1023s", cmd);
+    if (!strcmp(cmd, "xboard")) {
+      out("feature done=0\n");
+      xboard_mode = true;
+    } else if (!strcmp(cmd, "protover")) {
+      sscanf(line, "protover %d", &xboard_protover);
+      if (xboard_mode && xboard_protover < 2) {
+        out("Error (unsupported protocoll version,  < v2): protover\n");
+        out("tellusererror (unsupported protocoll version, < v2): protover\n");
+      } else {
+        out("feature myname=\"Zeta %s\"\n", VERSION);
+        out("feature ping=1\nfeature setboard=1\nfeature playother=0\nfeature san=0\nfeature usermove=1\n");
+        out("feature time=1\nfeature draw=0\nfeature sigint=0\nfeature reuse=1\nfeature analyze=0\n");
+        out("feature variants=\"normal\"\nfeature colors=0\nfeature ics=0\nfeature name=0\nfeature pause=0\n");
+        out("feature nps=0\nfeature debug=1\nfeature memory=1\nfeature smp=0\nfeature exclude=0\nfeature done=1\n");
+      }
+    } else if (!strcmp(cmd, "accepted") || !strcmp(cmd, "rejected")) {
+      if (strstr(line, "debug") && !strcmp(cmd, "accepted")) xboard_debug = true;
+    } else if (!strcmp(cmd, "new")) {
+      if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+      new_game(STARTFEN);
+      SD = MAXPLY;
+      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+      if (!xboard_mode) printboard(BOARD);
+      xboard_force = false;
+    } else if (!strcmp(cmd, "setboard")) {
+      const char *fen = line + strlen("setboard");
+      while (*fen == ' ') fen++;
+      new_game(fen);
+      if (!xboard_mode) printboard(BOARD);
+    } else if (!strcmp(cmd, "go")) {
+      xboard_force = false;
+      think();
+    } else if (!strcmp(cmd, "force")) {
+      xboard_force = true;
+    } else if (!strcmp(cmd, "level")) {
+      // level MPS BASE INC, BASE as minutes or minutes:seconds (src/xboard.c:384-425)
+      int mps = 0, mins = 0, secs = 0;
+      double inc = 0;
+      if (sscanf(line, "level %d %d:%d %lf", &mps, &mins, &secs, &inc) != 4) {
+        secs = 0;
+        sscanf(line, "level %d %d %lf", &mps, &mins, &inc);
+      }
+      TimeBase = (mins * 60.0 + secs) * 1000;
+      TimeInc = inc * 1000;
+      MaxMoves = mps > 0 ? mps : 1;
+      timemode = mps > 0 ? 1 : (inc > 0 ? 2 : 0);
+      MovesLeft = mps > 0 ? mps : 30;
+      TimeLeft = TimeBase;
+      MaxTime = TimeLeft / MovesLeft + TimeInc;
+      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+    } else if (!strcmp(cmd, "st")) {
+      double st = 0;
+      sscanf(line, "st %lf", &st);
+      timemode = 0;
+      MovesLeft = 1;
+      TimeBase = TimeLeft = MaxTime = st * 1000;
+      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+    } else if (!strcmp(cmd, "time")) {
+      double cs = 0;
+      sscanf(line, "time %lf", &cs);
+      TimeLeft = cs * 10;
+      MaxTime = TimeLeft / (MovesLeft > 0 ? MovesLeft : 1) + TimeInc;
+      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+    } else if (!strcmp(cmd, "otim") || !strcmp(cmd, "memory")) {
+    } else if (!strcmp(cmd, "usermove")) {
+      char mv[16] = {0};
+      sscanf(line, "usermove %15s", mv);
+      const uint32_t m = can2move(mv, BOARD, STM);
+      if (!m) {
+        out("Illegal move: %s\n", mv);
+        continue;
+      }
+      push_move(m);
+      if (!xboard_mode || xboard_debug) printboard(BOARD);
+      if (!xboard_force) think();
+    } else if (!strcmp(cmd, "undo")) {
+      if (PLY > 0) pop_move();
+    } else if (!strcmp(cmd, "remove")) {
+      if (PLY >= 2) {
+        pop_move();
+        pop_move();
+      }
+    } else if (!strcmp(cmd, "quit")) {
+      break;
+    } else if (!strcmp(cmd, "sd")) {
+      sscanf(line, "sd %d", &SD);
+      SD = SD > MAXPLY ? MAXPLY : SD;
+    } else if (!strcmp(cmd, "post")) {
+      xboard_post = true;
+    } else if (!strcmp(cmd, "nopost")) {
+      xboard_post = false;
+    } else if (!strcmp(cmd, "ping")) {
+      uint64_t pong = 0;
+      sscanf(line, "ping %" SCNu64, &pong);
+      out("pong %" PRIu64 "\n", pong);
+    } else if (!strcmp(cmd, "random") || !strcmp(cmd, "white") || !strcmp(cmd, "black") || !strcmp(cmd, "draw") ||
+               !strcmp(cmd, "result") || !strcmp(cmd, "hint") || !strcmp(cmd, "bk") || !strcmp(cmd, "hard") ||
+               !strcmp(cmd, "easy") || !strcmp(cmd, "name") || !strcmp(cmd, "rating") || !strcmp(cmd, "ics") ||
+               !strcmp(cmd, "computer")) {
+    } else if (!xboard_mode && !strcmp(cmd, "perft")) {
+      out("### doing inits for perft depth %d: ###\n", SD);
+      if (!cl_release_device() || !cl_init_device((char *)"perft_gpu")) quitengine(EXIT_FAILURE);
+      out("### computing perft depth %d: ###\n", SD);
+      const double start = now_ms();
+      perft(BOARD, STM, SD);
+      const double el = (now_ms() - start + 1) / 1000;
+      if (!cl_release_device()) quitengine(EXIT_FAILURE);
+      out("nodecount:%" PRIu64 ", seconds: %lf, nps: %" PRIu64 " \n", ABNODECOUNT, el,
+          (uint64_t)((double)ABNODECOUNT / el));
+      if (!cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+    } else if (!xboard_mode && !strcmp(cmd, "bench")) {
+      // fixed-depth search of the current position (time to depth and nps)
+      const double start = now_ms();
+      rootsearch(BOARD, STM, SD);
+      const double el = (now_ms() - start) / 1000;
+      out("#> bench: depth %d, %" PRIu64 " nodes in %lf seconds, nps: %" PRIu64 "\n", SD, ABNODECOUNT, el,
+          (uint64_t)(ABNODECOUNT / (el > 0 ? el : 1e-9)));
+    } else if (!xboard_mode && !strcmp(cmd, "selftest")) {
+      if (!cl_release_device() || !cl_init_device((char *)"perft_gpu")) quitengine(EXIT_FAILURE);
+      selftest();
+      if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+    } else if (!xboard_mode && !strcmp(cmd, "help")) {
+      print_help();
+    } else if (!xboard_mode && !strcmp(cmd, "log")) {
+      if (!LogFile) LogFile = fopen(LOGFILE, "a");
+      else {
+        fclose(LogFile);
+        LogFile = nullptr;
+      }
+    } else {
+      out("Error (unsupported command): %s\n", cmd);
+    }
+  }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  setvbuf(stdout, nullptr, _IONBF, 0);
+  for (int c = 1; c < argc; c++)
+    if (!strcmp(argv[c], "-l") || !strcmp(argv[c], "--log")) LogFile = fopen(LOGFILE, "a");
+  bool want_selftest = false;
+  for (int c = 1; c < argc; c++) {
+    const char *a = argv[c];
+    if (!strcmp(a, "-h") || !strcmp(a, "--help")) {
+      print_help();
+      return EXIT_SUCCESS;
+    }
+    if (!strcmp(a, "-v") || !strcmp(a, "--version")) {
+      out("Zeta version: %s\n", VERSION);
+      return EXIT_SUCCESS;
+    }
+    if (!strcmp(a, "-s") || !strcmp(a, "--selftest")) want_selftest = true;
+    if (!strcmp(a, "--guessconfig") || !strcmp(a, "--guessconfigx"))
+      return guess_config(!strcmp(a, "--guessconfigx")) ? EXIT_SUCCESS : EXIT_FAILURE;
+    if ((!strcmp(a, "-d") || !strcmp(a, "--d")) && c + 1 < argc) opencl_device_id = atoi(argv[++c]);
+    if (!strcmp(a, "--config") && c + 1 < argc) configfile = argv[++c];
+    if ((!strcmp(a, "-p") || !strcmp(a, "--p")) && c + 1 < argc) opencl_platform_id = atoi(argv[++c]);
+  }
+  out("#> Zeta %s\n#> Chess engine, HIP build for MI355X.\n#> engine is initialising...\n", VERSION);
+  out("feature done=0\n");
+  if (!read_and_init_config(configfile)) quitengine(EXIT_FAILURE);
+  if (!engineinits()) quitengine(EXIT_FAILURE);
+  new_game(STARTFEN);
+  if (want_selftest) {
+    if (!cl_init_device((char *)"perft_gpu")) quitengine(EXIT_FAILURE);
+    quitengine(selftest() ? EXIT_SUCCESS : EXIT_FAILURE);
+  }
+  if (!cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+  memcpy(GLOBAL_BOARD, BOARD, sizeof BOARD);
+  if (!cl_write_objects() || !cl_run_alphabeta(STM, 0, 1)) quitengine(EXIT_FAILURE);
+  out("#> ...finished basic inits.\n");
+  xboard();
+  quitengine(EXIT_SUCCESS);
+}

@@ -20,15 +20,30 @@ from . import (MLP_NPARAMS, mlp_get, mlp_init, mlp_set, mlp_train, playouts, rec
                records_reserve, search, search_counters)
 
 
-def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, chunk=4096):
-    """Log training pairs from plain searches of n playout positions (20-60 plies)."""
+def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, chunk=4096, progress=None):
+    """Log training pairs from plain searches of n playout positions (20-60 plies).
+
+    Cut nodes are sampled by hash (1 in 2^shift) so that the pairs spread over
+    all n positions instead of filling the buffer with the first ones; the
+    rate is set from a probe of the first 256 positions."""
     records_reserve(capacity)
-    records_clear()
     pos = playouts(seed, n, 20, 60)
     t = time.perf_counter()
+    records_clear()
+    probe = min(256, n)
+    search(pos[:probe], depth, workers=workers, tt_log2=26, clear_tt=True, log_records=True, log_shift=0)
+    per_pos = max(records_count(), 1) / probe
+    shift = 0
+    while shift < 20 and per_pos * n / (1 << shift) > 0.8 * capacity:
+        shift += 1
+    records_clear()
     for i in range(0, n, chunk):
-        search(pos[i:i + chunk], depth, workers=workers, tt_log2=26, clear_tt=(i == 0), log_records=True)
-    return {"positions": n, "depth": depth, "pairs": records_count(), "seconds": time.perf_counter() - t}
+        search(pos[i:i + chunk], depth, workers=workers, tt_log2=26, clear_tt=(i == 0), log_records=True,
+               log_shift=shift)
+        if progress:
+            progress({"generated_positions": min(i + chunk, n), "pairs": records_count()})
+    return {"positions": n, "depth": depth, "pairs": min(records_count(), capacity), "log_shift": shift,
+            "pairs_per_position_unsampled": per_pos, "seconds": time.perf_counter() - t}
 
 
 def train(steps=3000, pairs_per_step=2048, lr=1e-3, seed=1, init_seed=1):
