@@ -20,16 +20,23 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
+VALU_PEAK = 256 * 4 * 16 * 2.4e9  # VALU lane-instructions/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
 HBM_PEAK = 8.0e12
 
 
-# Algorithmic work of one leaf-kernel position (apply + analyse + count), in
-# 64-bit integer operations, counted by the op-counting host build of the
-# device core (tests/hostcore/opcount.cpp, `opcount 3` over 64 workload roots,
-# 2,115,331 leaf-level positions): bulk = full legal-move count of the node,
-# nobulk = the quiescence move count the reference does at each visited leaf.
-OPS_PER_POSITION = {"bulk": 931.5, "nobulk": 890.6}
+# Work of one k_leaf position (apply the parent's move, analyse, count the
+# legal moves), measured once with rocprofv3 --pmc on this workload and kept
+# as constants (profiles/perft_c2_pmc.md):
+#   SQ_INSTS_VALU = 2,959,784,086 wave instructions per launch of 128,916,248
+#   positions (one position per lane) -> 1469.4 VALU lane-instructions per
+#   position; each wave64 instruction occupies a 16-lane SIMD for 4 cycles,
+#   so the issue peak is 256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T/s.
+#   FETCH_SIZE 777,442.6 KiB x 1.99993 + WRITE_SIZE 63,073.4 KiB x 0.99999
+#   per launch, the factors from the library's copy kernel over 1 GiB
+#   (FETCH_SIZE 524,311.6 KiB, WRITE_SIZE 1,048,590.3 KiB) -> 12.85 B of HBM
+#   traffic per position.
+VALU_LANE_INSTS_PER_POSITION = 1469.4
+HBM_BYTES_PER_POSITION = 12.85
 
 
 def cpu_baseline(pos, depth, threads):
@@ -100,18 +107,17 @@ def main():
     # roofline of the dominant kernel, from its HIP-event time (separate timed pass)
     leaf = kst.get("perft_leaf", (0, 0.0, 0))
     roof = None
-    ops = OPS_PER_POSITION
-    if leaf[0]:
+    if leaf[0] and not args.nobulk:
         per_launch_items = leaf[2] / leaf[0]
         avg_s = leaf[1] / leaf[0] / 1e3
-        key = "nobulk" if args.nobulk else "bulk"
-        lane_ops = per_launch_items * ops[key] * 2  # 64-bit op = 2 int32 lane-ops
-        achieved = lane_ops / avg_s / 1e12
-        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK / 1e12, 2),
-                "unit": "Tops/s (int32 lane-ops)", "frac": round(achieved * 1e12 / VALU_PEAK, 4),
-                "traffic": None,
+        achieved = per_launch_items * VALU_LANE_INSTS_PER_POSITION / avg_s
+        roof = {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(VALU_PEAK / 1e12, 2),
+                "unit": "T VALU lane-instructions/s", "frac": round(achieved / VALU_PEAK, 4),
+                "traffic": int(per_launch_items * HBM_BYTES_PER_POSITION),
                 "kernel": "perft_leaf", "avg_launch_ms": round(leaf[1] / leaf[0], 4),
-                "items_per_launch": int(per_launch_items), "ops64_per_item": ops[key]}
+                "items_per_launch": int(per_launch_items),
+                "valu_lane_insts_per_item": VALU_LANE_INSTS_PER_POSITION,
+                "hbm_GBps": round(per_launch_items * HBM_BYTES_PER_POSITION / avg_s / 1e9, 1)}
 
     calib = None
     if args.calib_copy:

@@ -65,3 +65,40 @@ extern "C" uint64_t hc_perft(const uint64_t *b, int stm, int depth) {
   if (depth == 0) return 1;
   return perft_rec(load(b, stm), depth);
 }
+
+// the search's lane-per-square analysis, lanes emulated by a loop: attacked
+// squares combined as a ballot would, targets per origin square
+extern "C" int hc_enumerate_lane(const uint64_t *b, int stm, int qs, uint32_t *moves) {
+  Pos p = load(b, stm);
+  Node0 B;
+  node_basics(p, B);
+  u64 A = 0;
+  for (int sq = 0; sq < 64; sq++)
+    if (attacked_nk(B, sq)) A |= bit(sq);
+  int n = 0;
+  for (int sq = 0; sq < 64; sq++) {
+    if (!((B.own >> sq) & 1)) continue;
+    u64 t = lane_targets(p, B, A, sq, qs != 0);
+    while (t) {
+      int to = lsb(t);
+      t &= t - 1;
+      moves[n++] = make_code(p, sq, to);
+    }
+  }
+  return n;
+}
+
+extern "C" uint64_t hc_checkers(const uint64_t *b, int stm) {
+  Pos p = load(b, stm);
+  Node0 B;
+  node_basics(p, B);
+  return B.checkers;
+}
+
+extern "C" uint64_t hc_slider(int sq, uint64_t occ, int rook) {
+  return rook ? rook_sq(sq, occ) : bishop_sq(sq, occ);
+}
+
+extern "C" uint64_t hc_slider_ks(int sq, uint64_t occ, int rook) {
+  return rook ? rook_att(bit(sq), ~occ) : bishop_att(bit(sq), ~occ);
+}

@@ -26,6 +26,11 @@ def H():
     L.hc_enumerate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.hc_perft.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_perft.restype = ctypes.c_uint64
+    L.hc_enumerate_lane.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.hc_slider.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
+    L.hc_slider.restype = ctypes.c_uint64
+    L.hc_slider_ks.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
+    L.hc_slider_ks.restype = ctypes.c_uint64
     return L
 
 
@@ -72,3 +77,34 @@ def test_special_positions(H):
         b = np.ascontiguousarray(p["b"][0])
         for d in (1, 2, 3):
             assert H.hc_perft(b.ctypes.data, int(p["stm"][0]), d) == R.perft(p, d)
+
+
+def test_single_square_sliders_equal_fills(H):
+    # hyperbola attacks of one slider = the occluded fills, random occupancies
+    rng = np.random.default_rng(7)
+    for _ in range(3000):
+        occ = int(rng.integers(0, 2**63, dtype=np.int64)) & int(rng.integers(0, 2**63, dtype=np.int64))
+        sq = int(rng.integers(0, 64))
+        occ |= 1 << sq
+        for rook in (0, 1):
+            assert H.hc_slider(sq, occ, rook) == H.hc_slider_ks(sq, occ, rook), (sq, hex(occ), rook)
+
+
+def test_lane_analysis_matches_oracle(H):
+    # the search's lane-per-square generator: same legal moves, same order
+    pos = np.concatenate([R.gen_playouts(2, 1500, 8, 40), R.gen_playouts(3, 800, 20, 60),
+                          R.gen_playouts(9, 500, 60, 120)])
+    fens = ["R6R/3Q4/1Q4Q1/4Q3/2Q4Q/Q4Q2/pp1Q4/kBNN1KB1 w - - 0 1",
+            "r3k2r/8/8/8/8/8/8/R3K2R w KQkq - 0 1", "8/8/8/8/k2Pp2Q/8/8/3K4 b - - 0 1",
+            "8/8/8/2k5/3Pp3/8/8/4K2R b K - 0 1", "7k/5Q2/6K1/8/8/8/8/8 b - - 0 1",
+            "4k3/8/8/8/8/8/4r3/R3K2R w KQ - 0 1", "4k3/4r3/8/8/8/8/3B4/R3K2R w KQ - 0 1"]
+    pos = np.concatenate([pos] + [R.setboard(f) for f in fens])
+    buf = np.zeros(256, dtype=np.uint32)
+    ref = np.zeros(256, dtype=np.uint32)
+    for i in range(len(pos)):
+        b = np.ascontiguousarray(pos[i]["b"])
+        stm = int(pos[i]["stm"])
+        for qs in (0, 1):
+            n = H.hc_enumerate_lane(b.ctypes.data, stm, qs, buf.ctypes.data)
+            m = H.hc_enumerate(b.ctypes.data, stm, qs, ref.ctypes.data)
+            assert n == m and list(buf[:n]) == list(ref[:m]), (i, qs)

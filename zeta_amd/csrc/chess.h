@@ -177,6 +177,38 @@ ZD u64 anti_line(int k) {  // h1-a8 direction
   return s <= 7 ? (0x0102040810204080ULL >> (8 * (7 - s))) : (0x0102040810204080ULL << (8 * (s - 7)));
 }
 
+// ---- single-square slider attacks (hyperbola quintessence with a full
+// 64-bit bit reversal, which orders every line -- ranks included -- so the
+// borrow of a subtraction runs along it in both directions).  One slider
+// per lane: the search analyses one position with 64 lanes, lane = square.
+ZD u64 rev64(u64 x) {
+#if defined(__clang__)
+  return __builtin_bitreverse64(ZC_RAW(x));
+#else
+  u64 r = ZC_RAW(x);
+  r = ((r >> 1) & 0x5555555555555555ULL) | ((r & 0x5555555555555555ULL) << 1);
+  r = ((r >> 2) & 0x3333333333333333ULL) | ((r & 0x3333333333333333ULL) << 2);
+  r = ((r >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((r & 0x0F0F0F0F0F0F0F0FULL) << 4);
+  return __builtin_bswap64(r);
+#endif
+}
+// attacks of a slider on sq along the line `mask` (mask excludes sq)
+ZD u64 line_att(int sq, u64 occ, u64 mask) {
+  const u64 b = bit(sq);
+  const u64 o = occ & mask;
+  const u64 f = o - b;
+  const u64 r = rev64(rev64(o) - rev64(b));
+  return (f ^ r) & mask;
+}
+ZD u64 rook_sq(int sq, u64 occ) {
+  const u64 b = bit(sq);
+  return line_att(sq, occ, rank_line(sq) ^ b) | line_att(sq, occ, file_line(sq) ^ b);
+}
+ZD u64 bishop_sq(int sq, u64 occ) {
+  const u64 b = bit(sq);
+  return line_att(sq, occ, diag_line(sq) ^ b) | line_att(sq, occ, anti_line(sq) ^ b);
+}
+
 // ---- per-position legality context
 struct Legal {
   u64 own, opp, occ, kbb;
@@ -386,6 +418,142 @@ ZD u64 piece_targets(const Pos &p, const Legal &L, int sq, bool qs) {
     case QUEEN: return (rook_att(b, e) | bishop_att(b, e)) & T;
     default: return 0;
   }
+}
+
+// ---- node analysis with one lane per square (the search's wave-per-node
+// layout).  The uniform part is cheap (king-square attackers); everything
+// set-wise is replaced by per-square questions answered by all 64 lanes at
+// once and combined with a ballot:
+//   attacked(sq): is sq attacked by the opponent with the own king removed?
+//   the legal targets of the own piece on sq (pin line, check mask, king
+//   safety from the attacked-square mask).
+struct Node0 {
+  u64 own, opp, occ, kbb;
+  u64 pawns, knights, kings, rq, bq;
+  u64 checkers, cmask;
+  int ksq;
+  u32 stm;
+};
+
+ZD void node_basics(const Pos &p, Node0 &B) {
+  const u64 occ = p.b1 | p.b2 | p.b3;
+  const u64 white = p.b0 ^ occ;
+  B.stm = p.stm;
+  B.occ = occ;
+  B.own = p.stm ? p.b0 : white;
+  B.opp = p.stm ? white : p.b0;
+  B.pawns = p.b1 & ~p.b2 & ~p.b3;
+  B.knights = ~p.b1 & p.b2 & ~p.b3;
+  B.kings = p.b1 & p.b2 & ~p.b3;
+  B.rq = p.b3 & (p.b1 ^ p.b2);
+  B.bq = p.b3 & ~p.b1;
+  B.kbb = B.own & B.kings;
+  B.ksq = B.kbb ? lsb(B.kbb) : 0;
+  const u64 orq = B.opp & B.rq, obq = B.opp & B.bq;
+  u64 ch = (rook_sq(B.ksq, occ) & orq) | (bishop_sq(B.ksq, occ) & obq);
+  ch |= knight_att(B.kbb) & B.opp & B.knights;
+  ch |= pawn_att(B.kbb, p.stm) & B.opp & B.pawns;
+  ch = sel(B.kbb != 0, ch);
+  B.checkers = ch;
+  const int n = popc(ch);
+  u64 cm = 0;
+  if (n == 0) cm = FULL;
+  if (n == 1) {
+    // the checker, plus the squares between it and the king for a slider
+    const int c = lsb(ch);
+    const u64 r = rook_sq(B.ksq, occ) & rook_sq(c, occ);
+    const u64 d = bishop_sq(B.ksq, occ) & bishop_sq(c, occ);
+    const bool rl = (rank_line(B.ksq) & ch) || (file_line(B.ksq) & ch);
+    const bool slider = ((B.rq | B.bq) & ch) != 0;
+    cm = ch | sel(slider, rl ? r : d);
+  }
+  B.cmask = cm;
+}
+
+// is sq attacked by the opponent, the own king taken off the board?
+ZD bool attacked_nk(const Node0 &B, int sq) {
+  const u64 b = bit(sq);
+  const u64 occ = B.occ ^ B.kbb;
+  u64 a = (rook_sq(sq, occ) & B.rq) | (bishop_sq(sq, occ) & B.bq);
+  a |= knight_att(b) & B.knights;
+  a |= king_att(b) & B.kings;
+  a |= pawn_att(b, B.stm) & B.pawns;
+  return (a & B.opp) != 0;
+}
+
+// the line through the king and sq when the own piece on sq is pinned, else ~0
+ZD u64 pin_line(const Node0 &B, int sq) {
+  const int k = B.ksq;
+  const u64 b = bit(sq);
+  u64 m = 0, sl = 0;
+  if ((sq >> 3) == (k >> 3)) { m = rank_line(k); sl = B.rq; }
+  else if ((sq & 7) == (k & 7)) { m = file_line(k); sl = B.rq; }
+  else if (diag_line(k) & b) { m = diag_line(k); sl = B.bq; }
+  else if (anti_line(k) & b) { m = anti_line(k); sl = B.bq; }
+  if (!m || !B.kbb) return FULL;
+  const u64 me = m ^ bit(k);
+  const u64 seen = line_att(k, B.occ, me);
+  if (!(seen & b)) return FULL;  // not the first piece seen from the king
+  const u64 behind = line_att(k, B.occ ^ b, me) & ~seen;
+  return (behind & B.opp & sl) ? m : FULL;
+}
+
+ZD bool ep_legal0(const Node0 &B, int from, int to, int sqep) {
+  const u64 occ2 = (B.occ ^ bit(from) ^ bit(sqep)) | bit(to);
+  const u64 opp2 = B.opp & ~bit(sqep);
+  u64 att = (rook_sq(B.ksq, occ2) & opp2 & B.rq) | (bishop_sq(B.ksq, occ2) & opp2 & B.bq);
+  att |= knight_att(B.kbb) & opp2 & B.knights;
+  att |= pawn_att(B.kbb, B.stm) & opp2 & B.pawns;
+  att |= king_att(B.kbb) & opp2 & B.kings;
+  return att == 0;
+}
+
+// legal targets of the own piece on sq; A = squares attacked with the own
+// king transparent (only read for the king)
+ZD u64 lane_targets(const Pos &p, const Node0 &B, u64 A, int sq, bool qs) {
+  const u64 b = bit(sq);
+  const u32 type = (u32)((shrm(p.b1, sq) & 1) | ((shrm(p.b2, sq) & 1) << 1) | ((shrm(p.b3, sq) & 1) << 2));
+  const u64 own = B.own, occ = B.occ;
+  if (type == KING) {
+    u64 t = king_att(b) & (qs ? B.opp : ~own) & ~A;
+    if (!qs && (p.cr & SMCRALL)) {
+      const int k = sq;
+      const bool okq = p.stm ? ((~p.cr & SMCRBLACKQ) == SMCRBLACKQ) : ((~p.cr & SMCRWHITEQ) == SMCRWHITEQ);
+      const bool okk = p.stm ? ((~p.cr & SMCRBLACKK) == SMCRBLACKK) : ((~p.cr & SMCRWHITEK) == SMCRWHITEK);
+      const u64 ownrooks = own & B.rq & ~p.b2;
+      t |= sel(okq && (ownrooks & bit(k - 4)) && !(occ & (bit(k - 1) | bit(k - 2) | bit(k - 3))) &&
+                   !(A & (bit(k) | bit(k - 1) | bit(k - 2))),
+               bit(k - 2));
+      t |= sel(okk && (ownrooks & bit(k + 3)) && !(occ & (bit(k + 1) | bit(k + 2))) &&
+                   !(A & (bit(k) | bit(k + 1) | bit(k + 2))),
+               bit(k + 2));
+    }
+    return t;
+  }
+  const u64 pm = pin_line(B, sq);
+  const u64 T = (qs ? B.opp : ~own) & B.cmask & pm;
+  u64 t;
+  if (type == PAWN) {
+    t = pawn_att(b, p.stm) & B.opp & B.cmask & pm;
+    if (!qs) {
+      const u64 e = ~occ;
+      const u64 s1 = (p.stm ? (b >> 8) : (b << 8)) & e;
+      const u64 s2 = (p.stm ? ((s1 & RANK6) >> 8) : ((s1 & RANK3) << 8)) & e;
+      t |= (s1 | s2) & B.cmask & pm;
+    }
+    const u64 w = ~p.cr & RANKS45;
+    const int sqep = w ? lsb(w) : 0;
+    const bool cand = (p.stm ? (RANK4 & b) : (RANK5 & b)) && ((bit(sqep + 1) | bit(sqep - 1)) & b) && w;
+    if (cand) {
+      const int to = (p.stm ? sqep - 8 : sqep + 8) & 63;
+      if (ep_legal0(B, sq, to, sqep)) t |= bit(to);
+    }
+    return t;
+  }
+  // pieces: branch-free over the type (no divergence across lanes)
+  const bool rk = type == ROOK || type == QUEEN, bs = type == BISHOP || type == QUEEN;
+  t = sel(rk, rook_sq(sq, occ)) | sel(bs, bishop_sq(sq, occ)) | sel(type == KNIGHT, knight_att(b));
+  return t & T;
 }
 
 // Pack from->to with the reference movepicker rules (src/zetaperft.cl:1363-1386):

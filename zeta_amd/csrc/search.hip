@@ -787,11 +787,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               continue;
             }
           }
-          Legal L;
-          analyse(n.p, L);
-          // move generation, lane = square; a wave prefix sum compacts the lists
+          // node analysis and move generation, lane = square: the lanes answer
+          // "is my square attacked" (a ballot gives the attack map) and
+          // "which targets has my piece"; a wave prefix sum compacts the lists
+          Node0 B;
+          node_basics(n.p, B);
+          const u64 A = __ballot(attacked_nk(B, lane));
           u64 t = 0;
-          if ((L.own >> lane) & 1) t = piece_targets(n.p, L, lane, n.qs);
+          if ((B.own >> lane) & 1) t = lane_targets(n.p, B, A, lane, n.qs);
           int total;
           const int off = wave_exscan_i(popc(t), lane, total);
           n.mcnt = total;
@@ -822,6 +825,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           if (mlp_r && !n.qs && n.depth >= a.mlp_min_depth && total > 1) {
             wsync();
             mlp_score_moves(a, n.p, n.mbeg, total, lane, hpos, mmove, mscore);
+            // the network orders the quiet moves (no capture, no promotion,
+            // not a killer) among themselves and against the losing captures;
+            // captures, promotions and killers keep the plain scores
+            const u32 k0 = killers[sd][0], k1 = killers[sd][1];
+            for (int k = lane; k < total; k += 64) {
+              const u32 m = mmove[n.mbeg + k];
+              const bool quiet = ((mv_pcpt(m) >> 1) & 7) == PNONE && mv_pto(m) == mv_pfrom(m);
+              s32 sc;
+              if (quiet && m != k0 && m != k1) {
+                sc = (s32)mscore[n.mbeg + k] / 4;
+                sc = sc > 1250 ? 1250 : (sc < -1250 ? -1250 : sc);
+              } else {
+                sc = order_score(n.p.stm, m);
+                sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
+                if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
+                  if (m == k0) sc = KILLER0_SCORE;
+                  else if (m == k1) sc = KILLER1_SCORE;
+                }
+              }
+              mscore[n.mbeg + k] = (int16_t)sc;
+            }
+            wsync();
             c_mlp++;
           }
           // terminal / draw tests in the reference's order (src/zeta.cl:1554-1609)
@@ -834,7 +859,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             draw = __ballot(hit) != 0;
           }
           if (sd > 1 && n.hmc >= 100) draw = true;
-          if (sd > 1 && popc(L.occ) <= 2) draw = true;
+          if (sd > 1 && popc(B.occ) <= 2) draw = true;
           const bool need_eval = !draw && ((n.qs && !n.kic) || (n.qs && total == 0) || (sd >= MAXPLY && total > 0));
           s32 ev = 0;
           if (need_eval) {
