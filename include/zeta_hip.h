@@ -93,7 +93,7 @@ int zeta_playouts(uint64_t seed, int n, int minply, int maxply, zeta_pos *out);
  * order, TT and worker count.  nodes = moves made (src/zeta.cl:2283-2285). */
 typedef struct {
   int depth;          /* target depth, 1..40 */
-  int workers;        /* wavefront workers per root (ABDADA helpers), >= 1 */
+  int workers;        /* wavefront workers per root at most (ABDADA helpers), 1..1024 */
   int tt_log2;        /* TT size 2^tt_log2 x 16 B; -1 = zeta_config.tt_log2; 0 = none */
   int clear_tt;       /* 1 = clear the TT before the batch */
   uint64_t max_nodes; /* per-worker node budget per root, 0 = none */
@@ -114,7 +114,8 @@ typedef struct {
 int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, zeta_result *res);
 /* Counters of the last zeta_search_batch: [0] TT probes, [1] TT key hits,
  * [2] TT scores used, [3] TT stores, [4] ABDADA deferrals, [5] MAXPLY cuts,
- * [6] nodes ordered by the network, [7] training pairs logged.
+ * [6] nodes ordered by the network, [7] training pairs logged, [8] nodes
+ * made by all workers (after zeta_serve_stop: of the whole serving session).
  * Each probe / store moves one 16-byte entry. */
 int zeta_search_counters(uint64_t *out, int n);
 
@@ -172,7 +173,8 @@ int zeta_serve_poll(uint64_t ticket, zeta_result *res);
 int zeta_serve_reap(zeta_serve_done *out, int max, int *n);
 int zeta_serve_cancel(uint64_t ticket);
 /* cancels what is still running, waits for the grid to exit, frees the
- * slots; counters as zeta_search_counters; total_nodes may be NULL */
+ * slots; counters as zeta_search_counters; *total_nodes (may be NULL) = the
+ * nodes made by all workers, helpers included */
 int zeta_serve_stop(uint64_t *total_nodes);
 
 /* ------------------------------------------------ learned move ordering

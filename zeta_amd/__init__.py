@@ -216,10 +216,10 @@ def search_counters():
     """TT probes, TT hits, TT scores used, TT stores, ABDADA deferrals, MAXPLY
     cuts of the last search batch."""
     ensure_init()
-    out = np.zeros(8, dtype=np.uint64)
-    _check(lib().zeta_search_counters(_ptr(out), 8), "zeta_search_counters")
+    out = np.zeros(9, dtype=np.uint64)
+    _check(lib().zeta_search_counters(_ptr(out), 9), "zeta_search_counters")
     keys = ("tt_probes", "tt_hits", "tt_cuts", "tt_stores", "abdada_defers", "maxply_cuts", "mlp_nodes",
-            "records_logged")
+            "records_logged", "worker_nodes")
     return {k: int(out[i]) for i, k in enumerate(keys)}
 
 
@@ -429,7 +429,7 @@ class Server:
         _check(lib().zeta_serve_cancel(ticket), "zeta_serve_cancel")
 
     def stop(self):
-        """Stop the grid; returns the nodes of all collected and finished requests."""
+        """Stop the grid; returns the nodes made by all workers (helpers included)."""
         if not self.running:
             return 0
         tot = ctypes.c_uint64()

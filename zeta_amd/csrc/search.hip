@@ -82,20 +82,13 @@ struct ServeHost {  // pinned host memory header, then u64 queue[qcap], ack[qcap
   u32 cancel_gen;   // bumped when cancel[] changes
   u32 pad[10];
 };
-struct ServeDev {  // device memory; mirrors of the host words, refreshed by one worker at a time
+struct ServeDev {  // device memory; mirrors of the host words, written by the scheduler wavefront
   u64 claimed;     // tickets claimed by workers
   u64 sub;         // mirror of ServeHost::submitted
-  u64 hb;          // last heartbeat seen
-  u64 hb_time;     // wall clock when it last changed (0 = not yet)
-  u64 last_refresh;
   u32 shut;        // 1 shutdown, 2 watchdog, 3 protocol error
-  u32 poller;      // try-lock of the refresh
-  u32 cancel_gen;
-  u32 hb_xcc;      // XCC of the worker that timed the last heartbeat
-  u64 exit_now;    // diagnostics of a watchdog / protocol exit
+  u32 exit_xcc;    // diagnostics of a watchdog / protocol exit
+  u64 exit_now;
   u64 exit_arg;
-  u32 exit_xcc;
-  u32 pad;
 };
 // slot word (device): ticket+1 << 24 | max workers << 16 | workers << 2 | status
 enum { SV_RUN = 0, SV_DONE = 1, SV_FAILED = 2, SV_CANCELLED = 3 };
@@ -143,13 +136,12 @@ struct SearchArgs {
   ServeReq *dreq;     // device copies of the claimed requests
   u64 *dcancel;       // device mirror of scancel
   u64 watchdog_ticks; // wall-clock ticks without a heartbeat before the grid exits (0 = never)
-  u64 refresh_ticks;  // minimum wall-clock ticks between two refreshes of the mirrors
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_MLP_NODES, C_RECORDS, C_NCTR = 8 };
+enum { C_TT_PROBE = 0, C_TT_HIT, C_TT_CUT, C_TT_STORE, C_DEFER, C_MAXPLY, C_MLP_NODES, C_RECORDS, C_NODES, C_NCTR = 16 };
 
 // One ply of a worker's stack in LDS (64 B).  The board itself is not kept:
 // a parent board is recovered with undo() when a child returns.
@@ -484,70 +476,92 @@ __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u3
   }
 }
 
+// Serving synchronisation polls with relaxed atomics only: on this chip an
+// acquire or release at agent scope costs an L2 invalidate / write-back (the
+// L2 is per XCD), which, issued by polling workers, evicts every other
+// worker's transposition-table and weight lines.  Relaxed agent- and
+// system-scope atomics are coherent without cache maintenance.  Ordering is
+// paid only where data is handed over, once per request or join: the release
+// that opens a claimed request, the acquire of a helper's join, and the
+// release that publishes a finished result to the host.
 __device__ __forceinline__ u64 sys_load64(const u64 *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ u32 sys_load32(const u32 *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 template <class T> __device__ __forceinline__ T ag_load(const T *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class T> __device__ __forceinline__ void ag_store(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class T> __device__ __forceinline__ bool ag_cas(T *p, T expect, T v) {
-  return __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+  return __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void sys_store(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ u64 sw_tk(u64 v) { return v >> 24; }
 __device__ __forceinline__ int sw_max(u64 v) { return (int)((v >> 16) & 0xFF); }
 __device__ __forceinline__ int sw_w(u64 v) { return (int)((v >> 2) & 0x3FFF); }
 __device__ __forceinline__ int sw_st(u64 v) { return (int)(v & 3); }
 
-// refresh the device mirrors of the host words (lane 0; one worker at a time)
 __device__ __forceinline__ u32 xcc_id() {
   // HW_REG_XCC_ID (hwreg 20), bits [3:0]
   return (u32)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 15u;
 }
 
-__device__ void serve_refresh(const SearchArgs &a) {
+// The scheduler wavefront (the grid's last block) never searches: every few
+// microseconds it mirrors the host words (ticket count, shutdown, cancel
+// generation) into device memory, so that the workers poll device memory
+// only, and it runs the watchdog on its own clock.
+__device__ void serve_scheduler(const SearchArgs &a, int lane) {
   ServeDev *v = a.sv;
-  const u64 now = wall_clock64();
-  if ((long long)(now - ag_load(&v->last_refresh)) < (long long)a.refresh_ticks) return;
-  if (!ag_cas(&v->poller, 0u, 1u)) return;
-  ag_store(&v->last_refresh, now);
-  const u64 sub = sys_load64(&a.sh->submitted);
-  const u64 hb = sys_load64(&a.sh->heartbeat);
-  u32 shut = sys_load32(&a.sh->shutdown) ? 1u : 0u;
-  const u32 cg = sys_load32(&a.sh->cancel_gen);
-  if (hb != ag_load(&v->hb) || ag_load(&v->hb_time) == 0) {
-    ag_store(&v->hb, hb);
-    ag_store(&v->hb_time, now);
-    ag_store(&v->hb_xcc, xcc_id());
-  } else if (a.watchdog_ticks && (long long)(now - ag_load(&v->hb_time)) > (long long)a.watchdog_ticks) {
-    shut = 2;
-    ag_store(&v->exit_now, now);
-    ag_store(&v->exit_arg, ag_load(&v->hb_time));
-    ag_store(&v->exit_xcc, xcc_id() | (ag_load(&v->hb_xcc) << 8));
+  u64 hb_seen = 0, hb_time = 0;
+  u32 cg_seen = 0;
+  for (;;) {
+    u32 shut = 0, cg = 0;
+    if (lane == 0) {
+      const u64 now = wall_clock64();
+      const u64 sub = sys_load64(&a.sh->submitted);
+      const u64 hb = sys_load64(&a.sh->heartbeat);
+      shut = sys_load32(&a.sh->shutdown) ? 1u : 0u;
+      cg = sys_load32(&a.sh->cancel_gen);
+      if (hb != hb_seen || hb_time == 0) {
+        hb_seen = hb;
+        hb_time = now;
+      } else if (a.watchdog_ticks && now - hb_time > a.watchdog_ticks) {
+        shut = 2;
+        ag_store(&v->exit_now, now);
+        ag_store(&v->exit_arg, hb_time);
+        ag_store(&v->exit_xcc, xcc_id());
+      }
+      ag_store(&v->sub, sub);
+    }
+    cg = uf(cg);
+    if (cg != cg_seen) {
+      for (int s = lane; s < a.nslots; s += 64) ag_store(&a.dcancel[s], sys_load64(&a.scancel[s]));
+      cg_seen = cg;
+    }
+    shut = uf(shut);
+    if (shut) {
+      if (lane == 0) ag_store(&v->shut, shut);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(127);
   }
-  if (cg != ag_load(&v->cancel_gen)) {
-    for (int s = 0; s < a.nslots; s++) ag_store(&a.dcancel[s], sys_load64(&a.scancel[s]));
-    ag_store(&v->cancel_gen, cg);
-  }
-  ag_store(&v->sub, sub);
-  if (shut && !ag_load(&v->shut)) ag_store(&v->shut, shut);
-  ag_store(&v->poller, 0u);
 }
 
 // serving: claim the next published ticket, else join the running request
 // with the fewest workers; returns its slot and ticket+1, or -1 on shutdown
 __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
+  int backoff = 1;
   for (;;) {
     int res = -2;
     u64 tk = 0;
     if (lane == 0) {
-      serve_refresh(a);
       if (ag_load(&a.sv->shut)) {
         res = -1;
       } else {
@@ -563,15 +577,19 @@ __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
           } else {
             ServeReq &d = a.dreq[s];
             const ServeReq &h = a.slots[s].req;
-            for (int k = 0; k < 8; k++) d.pos[k] = h.pos[k];
-            d.depth = h.depth;
-            d.flags = h.flags;
-            d.workers = h.workers;
-            __hip_atomic_store(&a.sack[qi], c + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            for (int k = 0; k < 8; k++) ag_store(&d.pos[k], sys_load64(&h.pos[k]));
+            const s32 hw = (s32)sys_load32((const u32 *)&h.workers);
+            ag_store(&d.depth, (s32)sys_load32((const u32 *)&h.depth));
+            ag_store(&d.flags, (s32)sys_load32((const u32 *)&h.flags));
+            ag_store(&d.workers, hw);
+            sys_store(&a.sack[qi], c + 1);
             ag_store(&a.nodes[s], (u64)0);
             ag_store(&a.depth_done[s], 0);
-            const u64 wmax = (u64)(d.workers < 1 ? 1 : (d.workers > 255 ? 255 : d.workers));
-            ag_store(&a.sw[s], (u64)(((c + 1) << 24) | (wmax << 16) | (u64)4 | (u64)SV_RUN));
+            const u64 wmax = (u64)(hw < 1 ? 1 : (hw > 255 ? 255 : hw));
+            // release: the request copy above is visible to every helper
+            // that joins through this word (one write-back per request)
+            __hip_atomic_store(&a.sw[s], (u64)(((c + 1) << 24) | (wmax << 16) | (u64)4 | (u64)SV_RUN),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             res = (int)s;
             tk = c + 1;
           }
@@ -600,7 +618,10 @@ __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
       int got = -1;
       if (lane == 0) {
         const u64 v = ag_load(&a.sw[s]);
-        if (sw_tk(v) && sw_st(v) == SV_RUN && sw_w(v) >= 1 && sw_w(v) < sw_max(v) && ag_cas(&a.sw[s], v, v + 4)) {
+        u64 e = v;
+        if (sw_tk(v) && sw_st(v) == SV_RUN && sw_w(v) >= 1 && sw_w(v) < sw_max(v) &&
+            __hip_atomic_compare_exchange_strong(&a.sw[s], &e, v + 4, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {  // acquire: once per join
           got = s;
           tk = sw_tk(v);
         }
@@ -611,14 +632,16 @@ __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
         return got;
       }
     } else {
-      __builtin_amdgcn_s_sleep(32);
+      // nothing to do: back off (up to ~50 us) so idle workers stay off the
+      // memory system the searching ones use
+      for (int i = 0; i < backoff; i++) __builtin_amdgcn_s_sleep(127);
+      backoff = backoff < 16 ? 2 * backoff : 16;
     }
   }
 }
 
 // serving: abort test at the periodic poll
 __device__ __forceinline__ bool serve_abort(const SearchArgs &a, int r, u64 tk) {
-  serve_refresh(a);
   const u64 v = ag_load(&a.sw[r]);
   return sw_tk(v) != tk || sw_st(v) != SV_RUN || ag_load(&a.dcancel[r]) == tk || ag_load(&a.sv->shut) != 0;
 }
@@ -640,12 +663,13 @@ __device__ void serve_leave(const SearchArgs &a, int r, u64 tk, bool aborted, bo
     if (!ag_cas(&a.sw[r], v, nv)) continue;
     if (pub >= 0) {
       ServeSlot &sl = a.slots[r];
-      sl.score = pub == SV_DONE ? score : 0;
-      sl.bestmove = pub == SV_DONE ? move : 0u;
-      sl.nodes = ag_load(&a.nodes[r]);
-      sl.ticket = tk;
-      sl.depth_done = ag_load(&a.depth_done[r]);
-      sl.status = pub;
+      sys_store(&sl.score, pub == SV_DONE ? score : 0);
+      sys_store(&sl.bestmove, pub == SV_DONE ? move : 0u);
+      sys_store(&sl.nodes, ag_load(&a.nodes[r]));
+      sys_store(&sl.ticket, tk);
+      sys_store(&sl.depth_done, ag_load(&a.depth_done[r]));
+      sys_store(&sl.status, (s32)pub);
+      // the one ordering point: the result fields before the done flag
       __hip_atomic_store(&sl.state, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     return;
@@ -664,6 +688,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
 
   const bool serving = a.slots != nullptr;
+  if (serving && blockIdx.x == gridDim.x - 1) {
+    serve_scheduler(a, lane);
+    return;
+  }
   for (;;) {
     int r;
     u64 ticket = 0;
@@ -685,7 +713,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     }
 
     // ---------------------------------------------------------------- root set-up
-    const u64 *q = serving ? a.dreq[r].pos : a.roots + (size_t)r * 8;
+    u64 q[8];
+    if (serving) {
+      for (int k = 0; k < 8; k++) q[k] = uf(ag_load(&a.dreq[r].pos[k]));
+    } else {
+      for (int k = 0; k < 8; k++) q[k] = a.roots[(size_t)r * 8 + k];
+    }
     Pos root;
     root.b0 = q[0];
     root.b1 = q[1];
@@ -1057,6 +1090,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       atomicAdd((unsigned long long *)&a.ctr[C_MAXPLY], (unsigned long long)c_maxply);
       atomicAdd((unsigned long long *)&a.ctr[C_MLP_NODES], (unsigned long long)c_mlp);
       atomicAdd((unsigned long long *)&a.ctr[C_RECORDS], (unsigned long long)c_rec);
+      atomicAdd((unsigned long long *)&a.ctr[C_NODES], (unsigned long long)nodes);
       if (serving) {
         serve_leave(a, r, ticket, aborted, failed, final_score, final_move);
       } else {
@@ -1127,7 +1161,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   if (n < 0 || (n > 0 && (!pos || !res)) || !opts) return zr::fail(ZETA_EINVAL, "search: bad arguments");
   if (opts->depth < 1 || opts->depth > 40) return zr::fail(ZETA_EINVAL, "search: depth %d out of range", opts->depth);
   if (n == 0) return ZETA_OK;
-  const int workers = opts->workers < 1 ? 1 : (opts->workers > 64 ? 64 : opts->workers);
+  const int workers = opts->workers < 1 ? 1 : (opts->workers > 1024 ? 1024 : opts->workers);
   hipStream_t st = zr::ctx().stream;
   const int tt_log2 = opts->tt_log2 < 0 ? zr::ctx().cfg.tt_log2 : opts->tt_log2;
   const u64 tt_entries = tt_log2 > 0 ? (1ULL << tt_log2) : 0;
@@ -1137,7 +1171,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
       (r = zr::ensure(g_ss.done, (size_t)n * 4)) || (r = zr::ensure(g_ss.nworkers, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.score, (size_t)n * 4)) || (r = zr::ensure(g_ss.bestmove, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.nodes, (size_t)n * 8)) || (r = zr::ensure(g_ss.depth_done, (size_t)n * 4)) ||
-      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.abortf, 64)) || (r = zr::ensure(g_ss.ctr, 64)) ||
+      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.abortf, 64)) || (r = zr::ensure(g_ss.ctr, C_NCTR * 8)) ||
       (r = zr::ensure(g_ss.busy, (size_t)busy_entries * 4)))
     return r;
   if (tt_entries) {
@@ -1157,7 +1191,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   ZCHECK(hipMemsetAsync(g_ss.depth_done.p, 0, (size_t)n * 4, st));
   ZCHECK(hipMemsetAsync(g_ss.err.p, 0, 64, st));
   ZCHECK(hipMemsetAsync(g_ss.abortf.p, 0, 64, st));
-  ZCHECK(hipMemsetAsync(g_ss.ctr.p, 0, 64, st));
+  ZCHECK(hipMemsetAsync(g_ss.ctr.p, 0, C_NCTR * 8, st));
   ZCHECK(hipMemsetAsync(g_ss.busy.p, 0, (size_t)busy_entries * 4, st));
 
   int per_cu = 0;
@@ -1212,7 +1246,6 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.dreq = nullptr;
   sa.dcancel = nullptr;
   sa.watchdog_ticks = 0;
-  sa.refresh_ticks = 0;
   if (opts->log_records) {
     u64 cap = 0, *cnt = nullptr;
     void *recs = records_buffer(&cap, &cnt);
@@ -1310,7 +1343,6 @@ struct ServeState {
   std::vector<int> free_slots;
   u64 next_ticket = 0;
   u64 budget = 0, in_use = 0, tt_bytes = 0, worker_bytes = 0;
-  u64 total_nodes = 0;
   bool use_mlp = false;
 };
 ServeState g_sv;
@@ -1352,7 +1384,6 @@ void sv_take(int s, zeta_result *res) {
   res->nodes = sl.nodes;
   res->depth = sl.depth_done;
   res->status = sl.status == SV_DONE ? ZETA_OK : (sl.status == SV_FAILED ? ZETA_ENOMEM : ZETA_ECANCELED);
-  g_sv.total_nodes += sl.nodes;
   __atomic_store_n(&sl.state, 0u, __ATOMIC_RELEASE);
   g_sv.in_use -= g_sv.slot_cost[s];
   g_sv.slot_cost[s] = 0;
@@ -1428,7 +1459,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   if ((r = zr::ensure(S.sv, sizeof(ServeDev))) || (r = zr::ensure(S.sw, (size_t)S.nslots * 8)) ||
       (r = zr::ensure(S.dreq, (size_t)S.nslots * sizeof(ServeReq))) || (r = zr::ensure(S.dcancel, (size_t)S.nslots * 8)) ||
       (r = zr::ensure(S.nodes, (size_t)S.nslots * 8)) || (r = zr::ensure(S.depth_done, (size_t)S.nslots * 4)) ||
-      (r = zr::ensure(S.busy, (size_t)busy_entries * 4)) || (r = zr::ensure(S.ctr, 64)) || (r = zr::ensure(S.err, 64))) {
+      (r = zr::ensure(S.busy, (size_t)busy_entries * 4)) || (r = zr::ensure(S.ctr, C_NCTR * 8)) || (r = zr::ensure(S.err, 64))) {
     sv_free_all();
     return r;
   }
@@ -1456,7 +1487,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   SV_CHECK(hipMemsetAsync(S.nodes.p, 0, (size_t)S.nslots * 8, st));
   SV_CHECK(hipMemsetAsync(S.depth_done.p, 0, (size_t)S.nslots * 4, st));
   SV_CHECK(hipMemsetAsync(S.busy.p, 0, (size_t)busy_entries * 4, st));
-  SV_CHECK(hipMemsetAsync(S.ctr.p, 0, 64, st));
+  SV_CHECK(hipMemsetAsync(S.ctr.p, 0, C_NCTR * 8, st));
   SV_CHECK(hipMemsetAsync(S.err.p, 0, 64, st));
   SV_CHECK(hipStreamSynchronize(st));
 
@@ -1469,7 +1500,8 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   SV_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search, 64, 0));
   if (per_cu < 1) per_cu = 1;
   const int cap = per_cu * zr::ctx().cu_count;
-  const int grid = o->grid > 0 && o->grid < cap ? o->grid : cap;
+  int grid = o->grid > 0 && o->grid < cap ? o->grid : cap;
+  if (grid < 2) grid = 2;  // one scheduler wavefront + at least one worker
 
   SearchArgs sa;
   memset(&sa, 0, sizeof(sa));
@@ -1497,7 +1529,6 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   sa.dreq = (ServeReq *)S.dreq.p;
   sa.dcancel = (u64 *)S.dcancel.p;
   sa.watchdog_ticks = (u64)wd_ms * (u64)rate_khz;
-  sa.refresh_ticks = (u64)rate_khz / 100;  // 10 us
   S.free_slots.clear();
   for (int s = S.nslots - 1; s >= 0; s--) S.free_slots.push_back(s);
   S.slot_ticket.assign(S.nslots, 0);
@@ -1618,15 +1649,12 @@ extern "C" int zeta_serve_stop(uint64_t *total_nodes) {
   __atomic_store_n(&S.sh->shutdown, 1u, __ATOMIC_RELEASE);
   sv_heartbeat();
   hipError_t e = hipStreamSynchronize(zr::ctx().stream);
-  // requests finished but not collected count towards the total
-  for (int s = 0; s < S.nslots; s++)
-    if (S.slot_ticket[s] && __atomic_load_n(&S.slots[s].state, __ATOMIC_ACQUIRE)) S.total_nodes += S.slots[s].nodes;
   u64 ctr[C_NCTR] = {0};
   u32 err = 0;
   if (e == hipSuccess) e = hipMemcpy(ctr, S.ctr.p, sizeof(ctr), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(&err, S.err.p, 4, hipMemcpyDeviceToHost);
   if (e == hipSuccess) memcpy(g_ss.last_ctr, ctr, sizeof(ctr));
-  if (total_nodes) *total_nodes = S.total_nodes;
+  if (total_nodes) *total_nodes = ctr[C_NODES];
   sv_free_all();
   if (e != hipSuccess) return zr::hip_fail(e, "zeta_serve_stop", __FILE__, __LINE__);
   return ZETA_OK;

@@ -60,15 +60,16 @@ def run_stream(pos, depths, arrivals, learned=False, slots=256, workers=16, tt_l
             else:
                 time.sleep(0.0002)
         wall = time.perf_counter() - t0
+        worker_nodes = sv.stop()
     out = np.zeros(n, dtype=[("score", "<i4"), ("nodes", "<u8"), ("depth", "<i4"), ("status", "<i4")])
     for k, d in res.items():
         out[k] = (d["score"], d["nodes"], d["depth"], d["status"])
     lat = (finish - arrivals) * 1e3
-    return out, lat, wall
+    return out, lat, wall, worker_nodes
 
 
-def summarize(out, lat, wall, depths):
-    nodes = float(out["nodes"].sum())
+def summarize(out, lat, wall, depths, worker_nodes):
+    nodes = float(worker_nodes)
     per_depth = {}
     for d in np.unique(depths):
         m = depths == d
@@ -76,6 +77,7 @@ def summarize(out, lat, wall, depths):
                              "p99_ms": float(np.percentile(lat[m], 99))}
     return {"requests": len(out), "wall_s": round(wall, 3), "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "nodes": int(nodes), "nodes_per_s": nodes / wall,
+            "request_nodes": int(out["nodes"].sum()),
             "ok": int((out["status"] == 0).sum()), "by_depth": per_depth}
 
 
@@ -85,9 +87,9 @@ def compare(rate, n, seed=5, dmin=4, dmax=9, progress=None, **kw):
     rep = {"rate_per_s": rate, "requests": n, "depths": [dmin, dmax], "seed": seed}
     outs = {}
     for mode in ("plain", "learned"):
-        out, lat, wall = run_stream(pos, depths, arrivals, learned=(mode == "learned"), progress=progress, **kw)
+        out, lat, wall, wn = run_stream(pos, depths, arrivals, learned=(mode == "learned"), progress=progress, **kw)
         outs[mode] = out
-        rep[mode] = summarize(out, lat, wall, depths)
+        rep[mode] = summarize(out, lat, wall, depths, wn)
         if progress:
             progress({"rate": rate, "mode": mode, **{k: v for k, v in rep[mode].items() if k != "by_depth"}})
     rep["identical_root_scores"] = bool((outs["plain"]["score"] == outs["learned"]["score"]).all())
