@@ -37,6 +37,9 @@ HBM_PEAK = 8.0e12
 #   traffic per position.
 VALU_LANE_INSTS_PER_POSITION = 1469.4
 HBM_BYTES_PER_POSITION = 12.85
+# k_search: one node per wavefront; SQ_INSTS_VALU / nodes over the depth-5
+# search leg (profiles/search_c3_pmc.md)
+VALU_WAVE_INSTS_PER_NODE = 1264.0
 
 
 def cpu_baseline(pos, depth, threads):
@@ -153,6 +156,11 @@ def main():
                   "time_to_depth_s": round(ttd, 3), "nodes": nodes, "nodes_per_s": nodes / ttd,
                   "score_checksum": int(res["score"].astype(np.int64).sum()),
                   "tt_GBps": round(tt_bytes / ttd / 1e9, 3), "tt_frac_of_hbm": round(tt_bytes / ttd / HBM_PEAK, 6),
+                  "roofline": {"bound": "valu", "unit": "T VALU lane-instructions/s",
+                               "achieved": round(ctr["worker_nodes"] * VALU_WAVE_INSTS_PER_NODE * 64 / ttd / 1e12, 3),
+                               "peak": round(VALU_PEAK / 1e12, 2),
+                               "frac": round(ctr["worker_nodes"] * VALU_WAVE_INSTS_PER_NODE * 64 / ttd / VALU_PEAK, 4),
+                               "valu_wave_insts_per_node": VALU_WAVE_INSTS_PER_NODE},
                   **ctr}
 
     cpu = None

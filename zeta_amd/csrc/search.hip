@@ -431,20 +431,26 @@ __device__ void mlp_score_moves(const SearchArgs &a, const Pos &p, int mbeg, int
   wsync();
 }
 
-// training pairs at a cut node: the cutoff move against every move searched
-// before it (scores marked USED, the hard negatives) and against a
-// hash-selected sample of about four of the moves never searched, so nodes
-// whose first move cut contribute too; wave-compacted
+// training pairs at a cut node whose cutoff move is quiet (no capture, no
+// promotion, not a killer) -- the moves the learned ordering ranks: the
+// cutoff move against every quiet move searched before it (scores marked
+// USED, the hard negatives) and against a hash-selected sample of about two
+// quiet moves never searched; wave-compacted
+__device__ __forceinline__ bool plain_quiet(u32 m) {
+  return ((mv_pcpt(m) >> 1) & 7) == PNONE && mv_pto(m) == mv_pfrom(m);
+}
+
 __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u32 *mmove, const int16_t *mscore,
-                          u32 &nrec) {
-  const u32 thresh = n.mcnt > 4 ? (u32)(4 * 256 / n.mcnt) : 256u;
+                          u32 k0, u32 k1, u32 &nrec) {
+  if (!plain_quiet(n.move) || n.move == k0 || n.move == k1) return;
+  const u32 thresh = n.mcnt > 2 ? (u32)(2 * 256 / n.mcnt) : 256u;
   for (int c0 = 0; c0 < n.mcnt; c0 += 64) {
     const int k = c0 + lane;
     bool neg = false;
     u32 m = 0;
     if (k < n.mcnt) {
       m = mmove[n.mbeg + k];
-      if (m != n.move) {
+      if (m != n.move && m != k0 && m != k1 && plain_quiet(m)) {
         if (mscore[n.mbeg + k] == (int16_t)USED) {
           neg = true;
         } else {
@@ -859,16 +865,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             wsync();
             mlp_score_moves(a, n.p, n.mbeg, total, lane, hpos, mmove, mscore);
             // the network orders the quiet moves (no capture, no promotion,
-            // not a killer) among themselves and against the losing captures;
-            // captures, promotions and killers keep the plain scores
+            // not a killer) among themselves, in a band below every capture
+            // and killer, which keep the plain scores -- the plain ordering
+            // also ranks every capture (score >= 400) above every quiet move
             const u32 k0 = killers[sd][0], k1 = killers[sd][1];
             for (int k = lane; k < total; k += 64) {
               const u32 m = mmove[n.mbeg + k];
               const bool quiet = ((mv_pcpt(m) >> 1) & 7) == PNONE && mv_pto(m) == mv_pfrom(m);
               s32 sc;
               if (quiet && m != k0 && m != k1) {
-                sc = (s32)mscore[n.mbeg + k] / 4;
-                sc = sc > 1250 ? 1250 : (sc < -1250 ? -1250 : sc);
+                sc = (s32)mscore[n.mbeg + k] / 2;
+                sc = -16000 + (sc > 15000 ? 15000 : (sc < -15000 ? -15000 : sc));
               } else {
                 sc = order_score(n.p.stm, m);
                 sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
@@ -1051,7 +1058,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         if (n.alpha >= n.beta) {
           if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 1 && n.mcnt >= 2 &&
               ((u32)(n.hash >> 40) & a.rec_mask) == 0)
-            log_pairs(a, n, lane, mmove, mscore, c_rec);
+            log_pairs(a, n, lane, mmove, mscore, killers[sd][0], killers[sd][1], c_rec);
           if (((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0 && killers[sd][0] != n.move) {
             killers[sd][1] = killers[sd][0];
             killers[sd][0] = n.move;

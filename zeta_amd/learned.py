@@ -16,8 +16,8 @@ import time
 
 import numpy as np
 
-from . import (MLP_NPARAMS, mlp_get, mlp_init, mlp_set, mlp_train, playouts, records_clear, records_count,
-               records_reserve, search, search_counters)
+from . import (MLP_NPARAMS, mlp_get, mlp_grad, mlp_init, mlp_set, mlp_train, playouts, records_clear,
+               records_count, records_get, records_reserve, search, search_counters)
 
 
 def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, chunk=4096, progress=None):
@@ -46,12 +46,43 @@ def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, c
             "pairs_per_position_unsampled": per_pos, "seconds": time.perf_counter() - t}
 
 
-def train(steps=3000, pairs_per_step=2048, lr=1e-3, seed=1, init_seed=1):
+def heldout_pairs(seed=41, n=256, depth=4, workers=4, maxn=16384):
+    """Pairs from positions the training never sees (validation set)."""
+    generate_records(seed=seed, n=n, depth=depth, workers=workers, capacity=max(4 * maxn, 65536), chunk=n)
+    held = records_get()
+    return held[:(min(len(held), maxn) // 16) * 16]
+
+
+def train(steps=3000, pairs_per_step=2048, lr=2e-4, seed=1, init_seed=1, held=None, eval_every=100, patience=4):
+    """Adam on random batches of the logged pairs.  With validation pairs the
+    loss on them is measured every eval_every steps; training stops after
+    `patience` evaluations without improvement and keeps the best weights
+    (the network overfits the positions it was trained on quickly)."""
     mlp_init(init_seed)
     t = time.perf_counter()
-    losses = mlp_train(steps, pairs_per_step, lr, seed)
-    return {"steps": steps, "pairs_per_step": pairs_per_step, "lr": lr, "seconds": time.perf_counter() - t,
-            "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())}
+    if held is None:
+        losses = mlp_train(steps, pairs_per_step, lr, seed)
+        return {"steps": steps, "pairs_per_step": pairs_per_step, "lr": lr, "seconds": time.perf_counter() - t,
+                "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())}
+    best, best_params, best_step, bad = mlp_grad(held)[0], mlp_get(), 0, 0
+    first = None
+    done = 0
+    curve = []
+    while done < steps and bad < patience:
+        k = min(eval_every, steps - done)
+        losses = mlp_train(k, pairs_per_step, lr, seed + done)
+        if first is None:
+            first = float(losses[:20].mean())
+        done += k
+        hl = mlp_grad(held)[0]
+        curve.append([done, round(float(losses[-20:].mean()), 4), round(hl, 4)])
+        if hl < best:
+            best, best_params, best_step, bad = hl, mlp_get(), done, 0
+        else:
+            bad += 1
+    mlp_set(best_params)
+    return {"steps_run": done, "best_step": best_step, "pairs_per_step": pairs_per_step, "lr": lr,
+            "seconds": time.perf_counter() - t, "loss_first": first, "heldout_best": best, "curve": curve}
 
 
 def evaluate(seed=40, n=1024, depth=7, workers=1, mlp_min_depth=2, tt_log2=26):
@@ -93,7 +124,8 @@ def main():
     ap = argparse.ArgumentParser(description="learned move ordering: generate, train, evaluate")
     ap.add_argument("--train-positions", type=int, default=20000)
     ap.add_argument("--train-depth", type=int, default=5)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--eval-positions", type=int, default=1024)
     ap.add_argument("--eval-depth", type=int, default=7)
     ap.add_argument("--eval-workers", type=int, default=1)
@@ -105,9 +137,10 @@ def main():
     if args.load:
         load(args.load)
     else:
+        held = heldout_pairs()
         rep["records"] = generate_records(n=args.train_positions, depth=args.train_depth)
         print(json.dumps(rep["records"]), flush=True)
-        rep["training"] = train(steps=args.steps)
+        rep["training"] = train(steps=args.steps, lr=args.lr, held=held)
         print(json.dumps(rep["training"]), flush=True)
         if args.checkpoint:
             save(args.checkpoint)
