@@ -117,8 +117,9 @@ def test_serve_admission_budget(zeta, oracle, mixed):
 def test_serve_cancel_one_request(zeta, oracle, mixed):
     pos, depths, want = mixed
     deep = oracle.setboard("r1bqkb1r/pppp1ppp/2n2n2/4p3/2B1P3/5N2/PPPP1PPP/RNBQK2R w KQkq - 4 4")
-    with zeta.Server(slots=8, workers=2, tt_log2=20) as sv:
-        td = sv.submit(deep[0], 14, workers=1)
+    with zeta.Server(slots=8, workers=8, tt_log2=20) as sv:
+        # helpers join the deep request; none may join once it is cancelled
+        td = sv.submit(deep[0], 14, workers=8)
         assert td is not None
         tickets = [sv.submit(pos[k], int(depths[k])) for k in range(6)]
         assert all(t is not None for t in tickets)

@@ -614,7 +614,9 @@ __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
       if (s < a.nslots) {
         const u64 v = ag_load(&a.sw[s]);
         const int w = sw_w(v);
-        if (sw_tk(v) && sw_st(v) == SV_RUN && w >= 1 && w < sw_max(v))
+        // a cancelled request takes no new helpers: its workers drain and the
+        // last one publishes the cancellation
+        if (sw_tk(v) && sw_st(v) == SV_RUN && w >= 1 && w < sw_max(v) && ag_load(&a.dcancel[s]) != sw_tk(v))
           bestkey = max(bestkey, ((u32)(255 - w) << 16) | (u32)(0xFFFF - s));
       }
     }
@@ -626,6 +628,7 @@ __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
         const u64 v = ag_load(&a.sw[s]);
         u64 e = v;
         if (sw_tk(v) && sw_st(v) == SV_RUN && sw_w(v) >= 1 && sw_w(v) < sw_max(v) &&
+            ag_load(&a.dcancel[s]) != sw_tk(v) &&
             __hip_atomic_compare_exchange_strong(&a.sw[s], &e, v + 4, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)) {  // acquire: once per join
           got = s;

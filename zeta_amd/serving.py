@@ -25,7 +25,9 @@ def make_stream(rate, n, seed=5, dmin=4, dmax=9):
 
 
 def run_stream(pos, depths, arrivals, learned=False, slots=256, workers=16, tt_log2=26, mem_budget=0,
-               mlp_min_depth=2, progress=None):
+               mlp_min_depth=2, progress=None, max_seconds=None):
+    """Run the stream; with max_seconds, requests still running (or not yet
+    admitted) when it expires are cancelled and reported as such."""
     n = len(pos)
     finish = np.full(n, np.nan)
     res = {}
@@ -36,8 +38,19 @@ def run_stream(pos, depths, arrivals, learned=False, slots=256, workers=16, tt_l
                 mem_budget=mem_budget) as sv:
         t0 = time.perf_counter()
         last_report = t0
+        expired = False
         while len(res) < n:
             now = time.perf_counter() - t0
+            if max_seconds and not expired and now > max_seconds:
+                expired = True
+                for t in list(tick):
+                    sv.cancel(t)
+                for k in backlog:
+                    res[k] = None
+                backlog.clear()
+                while i < n:
+                    res[i] = None
+                    i += 1
             while i < n and arrivals[i] <= now:
                 backlog.append(i)
                 i += 1
@@ -63,22 +76,27 @@ def run_stream(pos, depths, arrivals, learned=False, slots=256, workers=16, tt_l
         worker_nodes = sv.stop()
     out = np.zeros(n, dtype=[("score", "<i4"), ("nodes", "<u8"), ("depth", "<i4"), ("status", "<i4")])
     for k, d in res.items():
-        out[k] = (d["score"], d["nodes"], d["depth"], d["status"])
+        out[k] = (0, 0, 0, -100) if d is None else (d["score"], d["nodes"], d["depth"], d["status"])
     lat = (finish - arrivals) * 1e3
     return out, lat, wall, worker_nodes
 
 
 def summarize(out, lat, wall, depths, worker_nodes):
+    """Latency percentiles over the completed requests (status 0)."""
     nodes = float(worker_nodes)
+    ok = out["status"] == 0
     per_depth = {}
     for d in np.unique(depths):
-        m = depths == d
-        per_depth[int(d)] = {"n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
-                             "p99_ms": float(np.percentile(lat[m], 99))}
-    return {"requests": len(out), "wall_s": round(wall, 3), "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "nodes": int(nodes), "nodes_per_s": nodes / wall,
-            "request_nodes": int(out["nodes"].sum()),
-            "ok": int((out["status"] == 0).sum()), "by_depth": per_depth}
+        m = (depths == d) & ok
+        if m.any():
+            per_depth[int(d)] = {"n": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                                 "p99_ms": float(np.percentile(lat[m], 99))}
+    return {"requests": len(out), "completed": int(ok.sum()), "cancelled": int((out["status"] == -7).sum()),
+            "not_admitted": int((out["status"] == -100).sum()), "wall_s": round(wall, 3),
+            "p50_ms": float(np.percentile(lat[ok], 50)) if ok.any() else None,
+            "p99_ms": float(np.percentile(lat[ok], 99)) if ok.any() else None,
+            "nodes": int(nodes), "nodes_per_s": nodes / wall, "request_nodes": int(out["nodes"].sum()),
+            "by_depth": per_depth}
 
 
 def compare(rate, n, seed=5, dmin=4, dmax=9, progress=None, **kw):
@@ -92,7 +110,9 @@ def compare(rate, n, seed=5, dmin=4, dmax=9, progress=None, **kw):
         rep[mode] = summarize(out, lat, wall, depths, wn)
         if progress:
             progress({"rate": rate, "mode": mode, **{k: v for k, v in rep[mode].items() if k != "by_depth"}})
-    rep["identical_root_scores"] = bool((outs["plain"]["score"] == outs["learned"]["score"]).all())
+    both = (outs["plain"]["status"] == 0) & (outs["learned"]["status"] == 0)
+    rep["identical_root_scores"] = bool((outs["plain"]["score"][both] == outs["learned"]["score"][both]).all())
+    rep["compared"] = int(both.sum())
     return rep
 
 
@@ -106,6 +126,8 @@ def main():
     ap.add_argument("--dmax", type=int, default=9)
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--max-seconds", type=float, default=None,
+                    help="cancel what is still running after this long (per stream and mode)")
     ap.add_argument("--net", default=None, help="trained network (npz); without it the learned run uses a "
                                                 "randomly initialised network")
     args = ap.parse_args()
@@ -117,7 +139,7 @@ def main():
     prog = lambda d: print(json.dumps(d), flush=True)  # noqa: E731
     for r in [float(x) for x in args.rates.split(",")]:
         rep = compare(r, args.requests, dmin=args.dmin, dmax=args.dmax, slots=args.slots, workers=args.workers,
-                      progress=prog)
+                      progress=prog, max_seconds=args.max_seconds)
         print(json.dumps(rep), flush=True)
 
 
