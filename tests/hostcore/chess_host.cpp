@@ -1,6 +1,7 @@
 // Host build of the device chess core (zeta_amd/csrc/chess.h) for CPU unit
 // tests of the kernel logic.  Test-only: the product never loads this.
 #include "../../zeta_amd/csrc/chess.h"
+#include "../../zeta_amd/csrc/evaltab.h"
 
 using namespace zc;
 
@@ -101,4 +102,12 @@ extern "C" uint64_t hc_slider(int sq, uint64_t occ, int rook) {
 
 extern "C" uint64_t hc_slider_ks(int sq, uint64_t occ, int rook) {
   return rook ? rook_att(bit(sq), ~occ) : bishop_att(bit(sq), ~occ);
+}
+
+// the search's static evaluation, lanes emulated by a loop: white positive
+extern "C" int32_t hc_eval(const uint64_t *b) {
+  Pos p = load(b, 0);
+  int32_t e = 0;
+  for (int sq = 0; sq < 64; sq++) e += eval_square(p, sq);
+  return e + eval_bishop_pair(p);
 }

@@ -27,6 +27,8 @@ def H():
     L.hc_perft.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_perft.restype = ctypes.c_uint64
     L.hc_enumerate_lane.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.hc_eval.argtypes = [vp]
+    L.hc_eval.restype = ctypes.c_int32
     L.hc_slider.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
     L.hc_slider.restype = ctypes.c_uint64
     L.hc_slider_ks.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
@@ -108,3 +110,15 @@ def test_lane_analysis_matches_oracle(H):
             n = H.hc_enumerate_lane(b.ctypes.data, stm, qs, buf.ctypes.data)
             m = H.hc_enumerate(b.ctypes.data, stm, qs, ref.ctypes.data)
             assert n == m and list(buf[:n]) == list(ref[:m]), (i, qs)
+
+
+def test_device_eval_matches_oracle(H):
+    # the search's per-square evaluation (one table lookup, pawn-file terms by
+    # popcount) = the oracle's evaluation of the reference's terms
+    pos = np.concatenate([R.gen_playouts(2, 1500, 8, 40), R.gen_playouts(3, 1500, 20, 60),
+                          R.gen_playouts(9, 500, 60, 120),
+                          R.setboard("4k3/8/3p4/3P4/3P4/3P4/8/4K3 w - - 0 1"),
+                          R.setboard("4k3/3p4/3p4/3p4/3P4/8/8/4K3 b - - 0 1")])
+    for i in range(len(pos)):
+        b = np.ascontiguousarray(pos[i]["b"])
+        assert H.hc_eval(b.ctypes.data) == R.evaluate(pos[i:i + 1]), i

@@ -8,13 +8,13 @@
 
 namespace zc {
 
-static ZC_CONST const int32_t EVAL_VALUE[7] = {0, 100, 400, 0, 400, 600, 1200};
+static constexpr int32_t EVAL_VALUE[7] = {0, 100, 400, 0, 400, 600, 1200};
 
-static ZC_CONST const int32_t EVAL_CONTROL[64] = {
+static constexpr int32_t EVAL_CONTROL[64] = {
     0, 0, 5, 5, 5, 5, 0, 0,   5, 0, 5, 5, 5, 5, 0, 5,   0, 0, 10, 5, 5, 10, 0, 0, 0, 5, 5, 10, 10, 5, 5, 0,
     0, 5, 5, 10, 10, 5, 5, 0, 0, 0, 10, 5, 5, 10, 0, 0, 0, 0, 5, 5, 5, 5, 0, 0,   0, 0, 5, 5, 5, 5, 0, 0};
 
-static ZC_CONST const int32_t EVAL_PST[7 * 64] = {
+static constexpr int32_t EVAL_PST[7 * 64] = {
     // none
     0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
     0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
@@ -51,31 +51,49 @@ static ZC_CONST const int32_t EVAL_PST[7 * 64] = {
 // pawn terms (blocked -15, chain +10 per diagonal-behind own pawn, -30 per
 // own pawn behind on the file).  The bishop pair (src/zeta.cl:1522-1531) is
 // added once per position by the caller.
+// 10 + material + piece-square + square control of a piece type on a square
+// (black view), folded into one table so a square costs one lookup
+struct EvalSq {
+  int32_t v[7 * 64];
+};
+constexpr EvalSq make_eval_sq() {
+  EvalSq t{};
+  for (int ty = 1; ty < 7; ty++)
+    for (int sq = 0; sq < 64; sq++) t.v[ty * 64 + sq] = 10 + EVAL_VALUE[ty] + EVAL_PST[ty * 64 + sq] + EVAL_CONTROL[sq];
+  return t;
+}
+static ZC_CONST const EvalSq EVAL_SQ = make_eval_sq();
+
 ZD int32_t eval_square(const Pos &p, int sq) {
   const u32 pc = piece_at(p, sq);
   const int type = (int)((pc >> 1) & 7);
   if (type == PNONE) return 0;
   const int color = (int)(pc & 1);
-  const u64 occ = p.b1 | p.b2 | p.b3;
-  const u64 pawns = p.b1 & ~p.b2 & ~p.b3;
-  const u64 me = color ? p.b0 : (p.b0 ^ occ);
-  const u64 opp = color ? (p.b0 ^ occ) : p.b0;
   const int sqf = color ? sq : (sq ^ 56);
-  int32_t s = 10 + EVAL_VALUE[type] + EVAL_PST[type * 64 + sqf] + EVAL_CONTROL[sqf];
+  int32_t s = EVAL_SQ.v[type * 64 + sqf];
   if (type == PAWN) {
+    const u64 occ = p.b1 | p.b2 | p.b3;
+    const u64 pawns = p.b1 & ~p.b2 & ~p.b3;
+    const u64 me = color ? p.b0 : (p.b0 ^ occ);
+    const u64 opp = color ? (p.b0 ^ occ) : p.b0;
     const u64 mp = pawns & me;
+    const u64 file = 0x0101010101010101ULL << (sq & 7);
     if (color == 0) {
       if ((sq >> 3) < 7 && (opp & bit(sq + 8))) s -= 15;
       if ((sq & 7) < 7 && (mp & bit(sq - 7))) s += 10;
       if ((sq & 7) > 0 && (mp & bit(sq - 9))) s += 10;
-      for (uint8_t t = (uint8_t)(sq - 8); t > 7; t = (uint8_t)(t - 8))
-        if (mp & bit(t)) s -= 30;
+      // own pawns behind on the file, ranks 2 .. rank-1 (the reference's loop
+      // runs t = sq-8, sq-16, .. while t > 7)
+      if (sq >= 8) s -= 30 * popc(mp & file & (bit(sq) - 1) & ~0xFFULL);
+      else
+        for (uint8_t t = (uint8_t)(sq - 8); t > 7; t = (uint8_t)(t - 8))
+          if (mp & bit(t)) s -= 30;
     } else {
       if ((sq >> 3) > 0 && (opp & bit(sq - 8))) s -= 15;
       if ((sq & 7) > 0 && (mp & bit(sq + 7))) s += 10;
       if ((sq & 7) < 7 && (mp & bit(sq + 9))) s += 10;
-      for (uint8_t t = (uint8_t)(sq + 8); t < 56; t = (uint8_t)(t + 8))
-        if (mp & bit(t)) s -= 30;
+      // ranks rank+1 .. 7 (t = sq+8, sq+16, .. while t < 56)
+      s -= 30 * popc(mp & file & ~((bit(sq) << 1) - 1) & ~(0xFFULL << 56));
     }
   }
   return color ? -s : s;

@@ -799,7 +799,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           const u64 key = tt_key(n.hash, n.p.cr);
           u32x4 tte = {0u, 0u, 0u, 0u};
           if (early) tte = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
-          n.kic = in_check(n.p);
+          // node analysis, uniform part (king-square attackers by single-square
+          // sliders): gives the in-check flag before the quiescence stand pat
+          Node0 B;
+          node_basics(n.p, B);
+          n.kic = B.checkers != 0;
           if (n.depth >= 0 && !parent_qs && (n.kic || is_promo(n.lastmove))) n.depth++;
           n.qs = n.depth <= 0;
           n.best = 0;
@@ -807,6 +811,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           n.ndefer = 0;
           n.searched = 0;
           n.mcnt = 0;
+          s32 ev0 = 0;
+          bool have_ev = false;
           if (n.qs && !n.kic) {
             // quiescence stand pat before any move generation: the reference
             // returns the static eval both on a stand-pat cutoff and when no
@@ -820,7 +826,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               continue;
             }
             const s32 e = wave_sum_i(eval_square(n.p, lane)) + eval_bishop_pair(n.p);
-            const s32 ev0 = n.p.stm ? -e : e;
+            ev0 = n.p.stm ? -e : e;
+            have_ev = true;
             if (ev0 >= n.beta || sd >= MAXPLY) {
               if (sd >= MAXPLY) c_maxply++;
               v = ev0;
@@ -829,11 +836,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               continue;
             }
           }
-          // node analysis and move generation, lane = square: the lanes answer
-          // "is my square attacked" (a ballot gives the attack map) and
-          // "which targets has my piece"; a wave prefix sum compacts the lists
-          Node0 B;
-          node_basics(n.p, B);
+          // move generation, lane = square: the lanes answer "is my square
+          // attacked" (a ballot gives the attack map) and "which targets has
+          // my piece"; a wave prefix sum compacts the lists
           const u64 A = __ballot(attacked_nk(B, lane));
           u64 t = 0;
           if ((B.own >> lane) & 1) t = lane_targets(n.p, B, A, lane, n.qs);
@@ -905,7 +910,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           if (sd > 1 && popc(B.occ) <= 2) draw = true;
           const bool need_eval = !draw && ((n.qs && !n.kic) || (n.qs && total == 0) || (sd >= MAXPLY && total > 0));
           s32 ev = 0;
-          if (need_eval) {
+          if (need_eval && have_ev) {
+            ev = ev0;
+          } else if (need_eval) {
             const s32 e = wave_sum_i(eval_square(n.p, lane)) + eval_bishop_pair(n.p);
             ev = n.p.stm ? -e : e;
           }
