@@ -101,7 +101,15 @@ typedef struct {
   int mlp_min_depth;  /* nodes with depth >= this use the learned ordering */
   int log_records;    /* record cutoff examples for training (zeta_records_*) */
   int log_shift;      /* log the cut nodes whose hash has its low log_shift bits (of bits 40..) zero */
+  /* Game history for repetitions across the root (the reference's
+   * GLOBAL_HASHHISTORY / PLY, src/zeta.cl:1560-1577): NULL, or n blocks of
+   * ZETA_HISTORY hashes, block i holding root i's predecessors newest first
+   * (entry 0 = the position one ply before the root), history_len[i] of them
+   * valid.  Without it only the searched path is checked. */
+  const uint64_t *history;
+  const int32_t *history_len;
 } zeta_search_opts;
+#define ZETA_HISTORY 128
 
 typedef struct {
   int32_t score;     /* centipawns, side to move; mate = +-(32000 - plies) */

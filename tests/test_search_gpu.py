@@ -61,3 +61,64 @@ def test_config3_depth6_golden(zeta, oracle):
     bad = np.nonzero(got["score"] != gold["score6"])[0]
     print("depth 6 mismatches", len(bad), "nodes", int(got["nodes"].sum()), flush=True)
     assert len(bad) == 0
+
+
+def _shuffle_games(oracle, n, seed):
+    """Random games ending in a there-and-back shuffle by both sides, so the
+    root repeats a position of its own game history."""
+    rng = np.random.default_rng(seed)
+    games = []
+    while len(games) < n:
+        p = oracle.setboard(oracle.STARTPOS)
+        line = [p]
+        ok = True
+        for _ in range(int(rng.integers(6, 30))):
+            k = oracle.count_legal(p)
+            if k == 0:
+                ok = False
+                break
+            p = oracle.domove(p, oracle.nth_legal(p, int(rng.integers(0, k))))
+            line.append(p)
+        if not ok:
+            continue
+        moves = []
+        for _ in range(2):  # a quiet piece move for each side, then back
+            cands = [oracle.nth_legal(p, i) for i in range(oracle.count_legal(p))]
+            cands = [m for m in cands if (m >> 26) & 15 == 0 and ((m >> 19) & 7) not in (0, 1)
+                     and (m >> 22) & 15 == (m >> 18) & 15]
+            if not cands:
+                ok = False
+                break
+            m = cands[int(rng.integers(0, len(cands)))]
+            moves.append(m)
+            p = oracle.domove(p, m)
+            line.append(p)
+        if not ok:
+            continue
+        for m in moves:  # undo both in order: same pieces back
+            back = [oracle.nth_legal(p, i) for i in range(oracle.count_legal(p))]
+            back = [b for b in back if (b & 63) == ((m >> 6) & 63) and ((b >> 6) & 63) == (m & 63)]
+            if not back:
+                ok = False
+                break
+            p = oracle.domove(p, back[0])
+            line.append(p)
+        if ok and oracle.count_legal(p) > 0 and int(p["b"][0][6]) < 100:
+            games.append(line)
+    return games
+
+
+def test_repetitions_against_game_history(zeta, oracle):
+    games = _shuffle_games(oracle, 48, 17)
+    roots = np.concatenate([g[-1] for g in games])
+    hashes = [[int(q["b"][0][5]) for q in g] for g in games]
+    for depth in (3, 4):
+        want = [oracle.search(g[-1], depth, hist=np.array(h, dtype=np.uint64), ply_init=len(h) - 1)[0]
+                for g, h in zip(games, hashes)]
+        plain = [oracle.search(g[-1], depth)[0] for g in games]
+        got = zeta.search(roots, depth, workers=2, tt_log2=20, history=[h[-2::-1] for h in hashes])
+        differ = sum(int(a != b) for a, b in zip(want, plain))
+        print("depth", depth, "history changes", differ, "of", len(games), "root scores", flush=True)
+        assert list(got["score"]) == want
+        assert (got["status"] == 0).all()
+    assert differ >= 1

@@ -123,6 +123,9 @@ struct SearchArgs {
   u64 *rec_count;
   u64 rec_cap;
   u32 rec_mask;  // a cut node is logged when (hash >> 40) & rec_mask == 0
+  // game history before each root, newest first (ZETA_HISTORY per root)
+  const u64 *hist;
+  const s32 *hlen;
   // serving mode (slots != nullptr)
   ServeHost *sh;
   const u64 *squeue;  // host: slot of ticket t at [t % qcap]
@@ -737,6 +740,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     root.stm = (u32)(q[7] & 1u);
     const u64 roothash = q[5];
     const u32 roothmc = (u32)(q[6] & 0xFF);
+    int hl = 0;
+    if (a.hist) {
+      hl = uf(a.hlen[r]);
+      hl = hl < 0 ? 0 : (hl > ZETA_HISTORY ? ZETA_HISTORY : hl);
+    }
     for (int k = lane; k < (MAXPLY + 2) * 2; k += 64) killers[k >> 1][k & 1] = 0;
     wsync();
 
@@ -900,10 +908,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           // terminal / draw tests in the reference's order (src/zeta.cl:1554-1609)
           bool draw = false;
           if (total > 0 && !n.qs && sd > 1) {
-            // in-path repetition against plies plyn-2, plyn-4, .. >= plyn-hmc
+            // repetition against plies plyn-2, plyn-4, .. >= plyn-hmc: the
+            // searched path, then the game history before the root
             bool hit = false;
             const int nn = plyn - 2 - 2 * lane;
-            if (nn >= 0 && nn >= plyn - (int)n.hmc) hit = ply[nn + 1].hash == n.hash;
+            if (nn >= plyn - (int)n.hmc) {
+              if (nn >= 0) hit = ply[nn + 1].hash == n.hash;
+              else if (-nn - 1 < hl) hit = a.hist[(size_t)r * ZETA_HISTORY + (-nn - 1)] == n.hash;
+            }
             draw = __ballot(hit) != 0;
           }
           if (sd > 1 && n.hmc >= 100) draw = true;
@@ -1151,7 +1163,7 @@ __global__ void k_clear_u64(u64 *p, size_t n) {
 }
 
 struct SearchState {
-  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf, ctr;
+  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf, ctr, hist, hlen;
   u64 tt_entries = 0;
   u64 last_ctr[C_NCTR] = {0};
 };
@@ -1168,7 +1180,7 @@ void search_release() {
   zr::release(g_ss.tt); zr::release(g_ss.busy); zr::release(g_ss.roots); zr::release(g_ss.queue);
   zr::release(g_ss.done); zr::release(g_ss.nworkers); zr::release(g_ss.score); zr::release(g_ss.bestmove);
   zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err); zr::release(g_ss.abortf);
-  zr::release(g_ss.ctr);
+  zr::release(g_ss.ctr); zr::release(g_ss.hist); zr::release(g_ss.hlen);
   g_ss.tt_entries = 0;
 }
 
@@ -1251,6 +1263,17 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.rec_count = nullptr;
   sa.rec_cap = 0;
   sa.rec_mask = (opts->log_shift > 0 && opts->log_shift < 24) ? ((1u << opts->log_shift) - 1u) : 0u;
+  sa.hist = nullptr;
+  sa.hlen = nullptr;
+  if (opts->history) {
+    if (!opts->history_len) return zr::fail(ZETA_EINVAL, "search: history without history_len");
+    if ((r = zr::ensure(g_ss.hist, (size_t)n * ZETA_HISTORY * 8)) || (r = zr::ensure(g_ss.hlen, (size_t)n * 4)))
+      return r;
+    ZCHECK(hipMemcpyAsync(g_ss.hist.p, opts->history, (size_t)n * ZETA_HISTORY * 8, hipMemcpyHostToDevice, st));
+    ZCHECK(hipMemcpyAsync(g_ss.hlen.p, opts->history_len, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    sa.hist = (const u64 *)g_ss.hist.p;
+    sa.hlen = (const s32 *)g_ss.hlen.p;
+  }
   sa.sh = nullptr;
   sa.squeue = nullptr;
   sa.sack = nullptr;

@@ -18,6 +18,8 @@ struct ShimState {
   bool fresh_tt = true;  // the TT is cleared on the first search after cl_init_device
   zeta_pos root{};
   bool have_root = false;
+  uint64_t hist[ZETA_HISTORY] = {0};  // game history before the root, newest first
+  int32_t hist_len = 0;
   // results of the last run, copied out by cl_read_memory
   bool have_result = false;
   uint64_t nodes = 0;
@@ -76,6 +78,11 @@ extern "C" bool cl_write_objects(void) {
   }
   memset(&g.root, 0, sizeof g.root);
   for (int i = 0; i < 7; i++) g.root.board[i] = GLOBAL_BOARD[i];
+  // worker 0's copy of the game history: entries 0..PLY-1 precede the root
+  // (src/search.c:58-63, src/zeta.cl:1560-1577)
+  g.hist_len = 0;
+  if (GLOBAL_HASHHISTORY)
+    for (int32_t k = 0; k < ZETA_HISTORY && PLY - 1 - k >= 0; k++) g.hist[g.hist_len++] = GLOBAL_HASHHISTORY[PLY - 1 - k];
   g.have_root = true;
   g.have_result = false;
   return true;
@@ -117,6 +124,8 @@ extern "C" bool cl_run_alphabeta(bool stm, int32_t depth, uint64_t nodes) {
     o.tt_log2 = -1;
     o.clear_tt = g.fresh_tt ? 1 : 0;
     o.max_nodes = nodes;
+    o.history = g.hist;
+    o.history_len = &g.hist_len;
     zeta_result r;
     memset(&r, 0, sizeof r);
     const int rc = zeta_search_batch(&g.root, 1, &o, &r);
