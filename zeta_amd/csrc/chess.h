@@ -221,17 +221,20 @@ struct Legal {
   u32 stm;
 };
 
-template <int D>
-ZD void king_ray(const Legal &L, u64 sl, u64 &checkers, u64 &chkray, u64 &pin) {
-  u64 empty = ~L.occ;
-  u64 ray = slide<D>(L.kbb, empty);
-  u64 hit = ray & L.occ;
-  bool chk = (hit & sl) != 0;
-  checkers |= sel(chk, hit);
-  chkray |= sel(chk, ray);
-  u64 oh = hit & L.own;
-  u64 hit2 = slide<D>(oh, empty) & L.occ;
-  pin |= sel((hit2 & sl) != 0, oh);
+// Checkers, check ray and pins along one line through the king (m = the line
+// without the king square, sl = the opponent sliders that move along it):
+// single-square attacks from the king give the first piece each way; with
+// the own pieces among them removed, the next pieces show the pinners.
+ZD void king_line(const Legal &L, u64 m, u64 sl, u64 &checkers, u64 &chkray, u64 &pin) {
+  const int k = L.ksq;
+  const u64 lo = m & (bit(k) - 1), up = m & ~lo;
+  const u64 att = line_att(k, L.occ, m);
+  const u64 chk = att & sl;
+  checkers |= chk;
+  chkray |= sel((chk & up) != 0, att & up) | sel((chk & lo) != 0, att & lo);
+  const u64 blk = att & L.own;
+  const u64 pn = line_att(k, L.occ ^ blk, m) & ~att & sl;
+  pin |= sel((pn & up) != 0, blk & up) | sel((pn & lo) != 0, blk & lo);
 }
 
 ZD void analyse(const Pos &p, Legal &L) {
@@ -256,14 +259,14 @@ ZD void analyse(const Pos &p, Legal &L) {
   L.A = A;
   u64 checkers = (pawn_att(L.kbb, p.stm) & L.opp & L.pawns) | (knight_att(L.kbb) & L.opp & L.knights);
   u64 chkray = 0, pr = 0, pf = 0, pd = 0, pa = 0;
-  king_ray<0>(L, orq, checkers, chkray, pf);
-  king_ray<1>(L, orq, checkers, chkray, pf);
-  king_ray<2>(L, orq, checkers, chkray, pr);
-  king_ray<3>(L, orq, checkers, chkray, pr);
-  king_ray<4>(L, obq, checkers, chkray, pd);
-  king_ray<7>(L, obq, checkers, chkray, pd);
-  king_ray<5>(L, obq, checkers, chkray, pa);
-  king_ray<6>(L, obq, checkers, chkray, pa);
+  if (L.kbb) {
+    const int k = L.ksq;
+    const u64 kb = bit(k);
+    king_line(L, file_line(k) ^ kb, orq, checkers, chkray, pf);
+    king_line(L, rank_line(k) ^ kb, orq, checkers, chkray, pr);
+    king_line(L, diag_line(k) ^ kb, obq, checkers, chkray, pd);
+    king_line(L, anti_line(k) ^ kb, obq, checkers, chkray, pa);
+  }
   L.checkers = checkers;
   L.pinR = pr;
   L.pinF = pf;
