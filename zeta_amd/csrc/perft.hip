@@ -19,6 +19,7 @@
 
 #include "chess.h"
 #include "runtime.h"
+#include "wave.h"
 
 using namespace zc;
 
@@ -176,10 +177,10 @@ __global__ __launch_bounds__(TPB) void k_apply(Front f, const u32 *__restrict__ 
   g.cnt[j] = (u32)count_legal(c, L, false);
 }
 
-__device__ __forceinline__ u64 wave_sum(u64 v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// sum over the wave (all lanes active); a wave's per-root share is at most
+// 64 x 218 legal moves, so 32-bit DPP sums suffice
+__device__ __forceinline__ u64 wave_sum(u32 v) {
+  return (u64)(u32)__builtin_amdgcn_readlane(zw::wave_scan_add((int)v), 63);
 }
 
 // ---- leaf level: apply + count, accumulate per root
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(TPB) void k_leaf(Front f, const u32 *__restrict__ p
     const u32 rl = __shfl(root, leader, 64);
     const bool mine = active && root == rl;
     const u64 mask = __ballot(mine);
-    const u64 s = wave_sum(mine ? v : 0ULL);
+    const u64 s = wave_sum(mine ? (u32)v : 0u);
     if (lane == leader) atomicAdd((unsigned long long *)&rootcnt[rl], (unsigned long long)s);
     pending &= ~mask;
     active = active && !mine;

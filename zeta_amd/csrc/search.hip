@@ -36,6 +36,7 @@
 #include "evaltab.h"
 #include "mlp.h"
 #include "runtime.h"
+#include "wave.h"
 
 using namespace zc;
 
@@ -164,41 +165,14 @@ __device__ __forceinline__ u32 uf(u32 v) { return (u32)__builtin_amdgcn_readfirs
 __device__ __forceinline__ s32 uf(s32 v) { return (s32)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ u64 uf(u64 v) { return ((u64)uf((u32)(v >> 32)) << 32) | (u64)uf((u32)v); }
 
-// Wave-wide inclusive scans on DPP (data-parallel primitives: a VALU operand
-// read from another lane of the same row, no LDS round trip): within each
-// 16-lane row by row shifts of 1, 2, 4, 8, then across rows by broadcasting
-// lane 15 into row 1 and 3 and lane 31 into rows 2 and 3.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ int dpp0(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xF, true);
-}
-__device__ __forceinline__ int wave_scan_add(int x) {
-  x += dpp0<0x111, 0xF>(x);  // row_shr:1
-  x += dpp0<0x112, 0xF>(x);  // row_shr:2
-  x += dpp0<0x114, 0xF>(x);  // row_shr:4
-  x += dpp0<0x118, 0xF>(x);  // row_shr:8
-  x += dpp0<0x142, 0xA>(x);  // row_bcast:15 into rows 1, 3
-  x += dpp0<0x143, 0xC>(x);  // row_bcast:31 into rows 2, 3
-  return x;
-}
-__device__ __forceinline__ u32 umax_(u32 a, u32 b) { return a > b ? a : b; }
-__device__ __forceinline__ u32 wave_scan_max(u32 x) {
-  x = umax_(x, (u32)dpp0<0x111, 0xF>((int)x));
-  x = umax_(x, (u32)dpp0<0x112, 0xF>((int)x));
-  x = umax_(x, (u32)dpp0<0x114, 0xF>((int)x));
-  x = umax_(x, (u32)dpp0<0x118, 0xF>((int)x));
-  x = umax_(x, (u32)dpp0<0x142, 0xA>((int)x));
-  x = umax_(x, (u32)dpp0<0x143, 0xC>((int)x));
-  return x;
-}
 __device__ __forceinline__ s32 wave_sum_i(s32 v) {
-  return (s32)__builtin_amdgcn_readlane(wave_scan_add(v), 63);
+  return (s32)__builtin_amdgcn_readlane(zw::wave_scan_add(v), 63);
 }
 __device__ __forceinline__ u32 wave_max_u32(u32 v) {
-  return (u32)__builtin_amdgcn_readlane((int)wave_scan_max(v), 63);
+  return (u32)__builtin_amdgcn_readlane((int)zw::wave_scan_max(v), 63);
 }
 __device__ __forceinline__ int wave_exscan_i(int v, int lane, int &total) {
-  const int x = wave_scan_add(v);
+  const int x = zw::wave_scan_add(v);
   total = __builtin_amdgcn_readlane(x, 63);
   return x - v;
 }
