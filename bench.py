@@ -167,6 +167,17 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         cpu = cpu_baseline(pos, args.depth, threads)
+        if search is not None:
+            # the oracle's alpha-beta (same node definition: moves made) on a
+            # bounded sample of the same positions
+            from oracle import refcpu
+            t3 = time.perf_counter()
+            _, cn = refcpu.search_batch(spos[:64], 5, threads)
+            dt3 = time.perf_counter() - t3
+            search["cpu_baseline"] = {"value": float(cn.sum()) / dt3, "unit": "nodes/s", "cores": threads,
+                                      "kind": "port",
+                                      "sample": "oracle search of the first 64 positions at depth 5 "
+                                                "(%d nodes, %.1f s, %d threads)" % (int(cn.sum()), dt3, threads)}
 
     out = {"metric": "perft nodes/s", "value": value, "unit": "leaves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",

@@ -15,7 +15,8 @@
  *        opencl_device_id                     HIP device ordinal here
  *   out: COUNTERS[totalWorkUnits * 64]  [w*64+1] nodes, [3] TT move hits,
  *                                       [4] TT score hits, [5] IID hits (0)
- *        PV[MAXPLY]                     PV[0] = root score (as Score), PV[1] = best move
+ *        PV[MAXPLY]                     PV[0] = root score (as Score), PV[1..] = best move and
+ *                                       the principal variation read from the TT
  *
  * Differences, by design: the kernel name passed to cl_init_device selects
  * nothing (one library serves perft and search); cl_run_alphabeta(stm, d,

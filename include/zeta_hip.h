@@ -108,6 +108,11 @@ typedef struct {
    * valid.  Without it only the searched path is checked. */
   const uint64_t *history;
   const int32_t *history_len;
+  /* Principal variation (NULL = none): n rows of pv_len move codes, the best
+   * move then the moves the transposition table holds along it, 0-padded
+   * (the reference's PV[1..], src/zeta.cl:2450-2512). */
+  uint32_t *pv;
+  int pv_len;
 } zeta_search_opts;
 #define ZETA_HISTORY 128
 

@@ -122,3 +122,20 @@ def test_repetitions_against_game_history(zeta, oracle):
         assert list(got["score"]) == want
         assert (got["status"] == 0).all()
     assert differ >= 1
+
+
+def test_principal_variation_is_legal(zeta, oracle):
+    pos = oracle.gen_playouts(3, 32, 20, 60)
+    res, pv = zeta.search(pos, 5, workers=2, tt_log2=22, pv_len=8)
+    for i in range(len(pos)):
+        assert int(pv[i][0]) == int(res["bestmove"][i])
+        p = pos[i:i + 1]
+        for m in pv[i]:
+            if not m:
+                break
+            legal = [oracle.nth_legal(p, k) for k in range(oracle.count_legal(p))]
+            assert int(m) in legal
+            p = oracle.domove(p, int(m))
+    mate = oracle.setboard("6k1/5ppp/8/8/8/8/5PPP/R5K1 w - - 0 1")
+    res, pv = zeta.search(mate, 3, tt_log2=20, pv_len=4)
+    assert oracle.move2can(int(pv[0][0])) == "a1a8"

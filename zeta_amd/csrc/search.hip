@@ -1151,12 +1151,56 @@ __global__ __launch_bounds__(64) void k_mlp_score(SearchArgs a, const u64 *pos, 
   for (int k = lane; k < n; k += 64) out[b + k] = (float)mscore[k] / 256.f;
 }
 
+// Principal variation after a search: the best root move, then the moves the
+// transposition table holds for the positions that follow, each checked to
+// be legal (the reference walks its TT the same way, src/zeta.cl:2450-2512).
+// One thread per root; stops at a missing or mismatching entry.
+__global__ void k_pv(const u64 *roots, const u32 *bestmove, int n, int len, const u64 *tt, u64 ttmask,
+                     u32 *pv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 *out = pv + (size_t)i * len;
+  for (int k = 0; k < len; k++) out[k] = 0;
+  const u64 *q = roots + (size_t)i * 8;
+  Pos p;
+  p.b0 = q[0];
+  p.b1 = q[1];
+  p.b2 = q[2];
+  p.b3 = q[3];
+  p.cr = q[4];
+  p.stm = (u32)(q[7] & 1u);
+  Node nd;
+  nd.hash = q[5];
+  nd.phash = hash_pieces(p);
+  u32 m = bestmove[i];
+  for (int k = 0; k < len && m; k++) {
+    const int f = (int)mv_from(m), t = (int)mv_to(m);
+    Legal L;
+    analyse(p, L);
+    if (!((L.own >> f) & 1) || !((piece_targets(p, L, f, false) >> t) & 1) || make_code(p, f, t) != m) break;
+    out[k] = m;
+    const Pos c = apply(p, m, false);
+    u64 h, ph;
+    child_hash(nd, m, c, h, ph);
+    nd.hash = h;
+    nd.phash = ph;
+    p = c;
+    m = 0;
+    if (ttmask) {
+      const u64 key = tt_key(h, c.cr);
+      const u64 w0 = tt[2 * (key & ttmask)], w1 = tt[2 * (key & ttmask) + 1];
+      if ((w0 ^ w1) == key && ((w1 >> 57) & 1)) m = (u32)w1;
+    }
+  }
+}
+
 __global__ void k_clear_u64(u64 *p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 struct SearchState {
-  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf, ctr, hist, hlen;
+  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf, ctr, hist, hlen,
+      pv;
   u64 tt_entries = 0;
   u64 last_ctr[C_NCTR] = {0};
 };
@@ -1173,7 +1217,7 @@ void search_release() {
   zr::release(g_ss.tt); zr::release(g_ss.busy); zr::release(g_ss.roots); zr::release(g_ss.queue);
   zr::release(g_ss.done); zr::release(g_ss.nworkers); zr::release(g_ss.score); zr::release(g_ss.bestmove);
   zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err); zr::release(g_ss.abortf);
-  zr::release(g_ss.ctr); zr::release(g_ss.hist); zr::release(g_ss.hlen);
+  zr::release(g_ss.ctr); zr::release(g_ss.hist); zr::release(g_ss.hlen); zr::release(g_ss.pv);
   g_ss.tt_entries = 0;
 }
 
@@ -1308,6 +1352,18 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
     res[i].nodes = nodes[i];
     res[i].depth = dd[i];
     res[i].status = done[i] == 1 ? 0 : (done[i] == 2 ? ZETA_ENOMEM : ZETA_ESTATE);
+  }
+  if (opts->pv && opts->pv_len > 0) {
+    const int len = opts->pv_len > MAXPLY ? MAXPLY : opts->pv_len;
+    if ((r = zr::ensure(g_ss.pv, (size_t)n * len * 4))) return r;
+    hipLaunchKernelGGL(k_pv, dim3((n + 63) / 64), dim3(64), 0, st, (const u64 *)g_ss.roots.p,
+                       (const u32 *)g_ss.bestmove.p, n, len, (const u64 *)g_ss.tt.p, sa.ttmask, (u32 *)g_ss.pv.p);
+    ZCHECK(hipGetLastError());
+    std::vector<u32> pvh((size_t)n * len);
+    ZCHECK(hipMemcpyAsync(pvh.data(), g_ss.pv.p, pvh.size() * 4, hipMemcpyDeviceToHost, st));
+    ZCHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < opts->pv_len; k++) opts->pv[(size_t)i * opts->pv_len + k] = k < len ? pvh[(size_t)i * len + k] : 0u;
   }
   if (err & 1u) return zr::fail(ZETA_ENOMEM, "search: move stack overflow (more than %d moves on one path)", MSTACK);
   for (int i = 0; i < n; i++)

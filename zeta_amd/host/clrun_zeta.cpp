@@ -27,6 +27,7 @@ struct ShimState {
   uint64_t budget = 0;
   int32_t score = 0;
   uint32_t bestmove = 0;
+  uint32_t pv[MAXPLY - 1] = {0};
   uint64_t tt_hits = 0, tt_cuts = 0;
 };
 ShimState g;
@@ -96,6 +97,7 @@ extern "C" bool cl_run_perft(bool stm, int32_t depth) {
   g.budget = 0;
   g.score = 0;
   g.bestmove = 0;
+  memset(g.pv, 0, sizeof g.pv);
   g.tt_hits = g.tt_cuts = 0;
   if (depth >= 1) {
     uint64_t count = 0;
@@ -115,6 +117,7 @@ extern "C" bool cl_run_alphabeta(bool stm, int32_t depth, uint64_t nodes) {
   g.budget = nodes;
   g.score = 0;
   g.bestmove = 0;
+  memset(g.pv, 0, sizeof g.pv);
   g.tt_hits = g.tt_cuts = 0;
   if (depth >= 1) {  // the start-up call runs depth 0 (src/zeta.c:695)
     zeta_search_opts o;
@@ -126,6 +129,8 @@ extern "C" bool cl_run_alphabeta(bool stm, int32_t depth, uint64_t nodes) {
     o.max_nodes = nodes;
     o.history = g.hist;
     o.history_len = &g.hist_len;
+    o.pv = g.pv;
+    o.pv_len = MAXPLY - 1;
     zeta_result r;
     memset(&r, 0, sizeof r);
     const int rc = zeta_search_batch(&g.root, 1, &o, &r);
@@ -176,6 +181,7 @@ extern "C" bool cl_read_memory(void) {
   memset(PV, 0, sizeof(uint32_t) * MAXPLY);
   PV[0] = (uint32_t)g.score;
   PV[1] = g.bestmove;
+  for (int k = 1; k < MAXPLY - 1 && g.pv[k]; k++) PV[k + 1] = g.pv[k];
   return true;
 }
 

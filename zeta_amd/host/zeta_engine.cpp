@@ -311,8 +311,13 @@ uint32_t rootsearch(const uint64_t *board, bool stm, int32_t depth) {
       if (bestscore <= -MATESCORE) xs = -100000 - (INF + bestscore);
       if (bestscore >= MATESCORE) xs = 100000 - (-INF + bestscore);
       if ((xboard_post || !xboard_mode) && pvmove) {
-        out("%i %i %i %" PRIu64 " %s \n", idf, xs, (int32_t)(elapsed * 100), ABNODECOUNT,
-            move2can(PV[1]).c_str());
+        // thinking line with the principal variation (src/search.c:150-168)
+        std::string line;
+        for (int i = 1; i < MAXPLY && i <= idf; i++) {
+          if (!(PV[i] & 0x3FFFFFFF)) break;
+          line += move2can(PV[i]) + " ";
+        }
+        out("%i %i %i %" PRIu64 " %s\n", idf, xs, (int32_t)(elapsed * 100), ABNODECOUNT, line.c_str());
       } else {
         break;
       }
