@@ -59,3 +59,13 @@ def test_think_and_reply_in_a_game():
     assert p.returncode == 0
     moves = [line for line in p.stdout.splitlines() if line.startswith("move ")]
     assert len(moves) == 2
+
+
+def test_benchsmp_time_to_depth_per_worker_count():
+    p = run("sd 5\nbenchsmp\nquit\n")
+    assert p.returncode == 0
+    assert "# overall results" in p.stdout
+    rows = [line for line in p.stdout.splitlines() if line.startswith("### ") and "workers" not in line]
+    assert [int(r.split()[1]) for r in rows] == [1, 2, 4, 8, 16]
+    scores = {line.split("score: ")[1].split()[0] for line in p.stdout.splitlines() if line.startswith("depth: 5,")}
+    assert len(scores) == 1  # the worker count never changes the root score

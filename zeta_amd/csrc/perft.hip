@@ -141,7 +141,7 @@ __global__ __launch_bounds__(TPB) void k_scan_down(const u32 *__restrict__ cnt, 
 
 // ---- expand: (parent, move) pairs of every legal move, at the scanned offsets
 __global__ __launch_bounds__(TPB) void k_expand(Front f, size_t i0, size_t i1, const u64 *__restrict__ off,
-                                                u64 base, u32 *__restrict__ par, u32 *__restrict__ mv) {
+                                                u64 base, u64 *__restrict__ pm) {
   const size_t i = i0 + (size_t)blockIdx.x * TPB + threadIdx.x;
   if (i >= i1) return;
   const Pos p = load_pos(f, i);
@@ -156,21 +156,21 @@ __global__ __launch_bounds__(TPB) void k_expand(Front f, size_t i0, size_t i1, c
     while (t) {
       const int to = lsb(t);
       t &= t - 1;
-      par[o] = (u32)i;
-      mv[o] = make_code(p, sq, to);
+      pm[o] = ((u64)i << 32) | make_code(p, sq, to);  // parent index : move, one 8-byte store
       o++;
     }
   }
 }
 
 // ---- apply: materialise the next level and count its moves
-__global__ __launch_bounds__(TPB) void k_apply(Front f, const u32 *__restrict__ par, const u32 *__restrict__ mv,
+__global__ __launch_bounds__(TPB) void k_apply(Front f, const u64 *__restrict__ pm,
                                                size_t m, Front g) {
   const size_t j = (size_t)blockIdx.x * TPB + threadIdx.x;
   if (j >= m) return;
-  const u32 pi = par[j];
+  const u64 e = pm[j];
+  const u32 pi = (u32)(e >> 32);
   const Pos p = load_pos(f, pi);
-  const Pos c = apply(p, mv[j], true);
+  const Pos c = apply(p, (u32)e, true);
   store_pos(g, j, c, f.meta[pi] >> 1);
   Legal L;
   analyse(c, L);
@@ -184,16 +184,17 @@ __device__ __forceinline__ u64 wave_sum(u32 v) {
 }
 
 // ---- leaf level: apply + count, accumulate per root
-__global__ __launch_bounds__(TPB) void k_leaf(Front f, const u32 *__restrict__ par, const u32 *__restrict__ mv,
+__global__ __launch_bounds__(TPB) void k_leaf(Front f, const u64 *__restrict__ pm,
                                               size_t m, u64 *__restrict__ rootcnt, int nobulk) {
   const size_t j = (size_t)blockIdx.x * TPB + threadIdx.x;
   bool active = j < m;
   u64 v = 0;
   u32 root = 0;
   if (active) {
-    const u32 pi = par[j];
+    const u64 e = pm[j];
+    const u32 pi = (u32)(e >> 32);
     const Pos p = load_pos(f, pi);
-    const Pos c = apply(p, mv[j], true);
+    const Pos c = apply(p, (u32)e, true);
     Legal L;
     analyse(c, L);
     if (nobulk) {
@@ -295,7 +296,7 @@ constexpr int MAXLEV = 16;
 struct LevelBufs {
   zr::DevBuf b0, b1, b2, b3, cr, meta, cnt;  // SoA positions
   zr::DevBuf off, bsum;                      // scan
-  zr::DevBuf par, mv;                        // pairs of the next level
+  zr::DevBuf pm;                             // (parent index << 32 | move) pairs of the next level
   size_t cap = 0, paircap = 0;
 };
 
@@ -324,7 +325,7 @@ int ensure_pairs(int L, size_t m) {
   if (b.paircap >= m && b.paircap) return ZETA_OK;
   size_t cap = m + m / 8 + 1024;
   int r;
-  if ((r = zr::ensure(b.par, cap * 4)) || (r = zr::ensure(b.mv, cap * 4))) return r;
+  if ((r = zr::ensure(b.pm, cap * 8))) return r;
   b.paircap = cap;
   return ZETA_OK;
 }
@@ -384,15 +385,14 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
       const u64 m = off[a1] - off[a0];
       if ((r = ensure_pairs(L, m))) return r;
       ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, TPB), TPB, 0, front_of(L), i0 + a0, i0 + a1,
-              (const u64 *)g_ps.lev[L].off.p - i0, (u64)off[a0], (u32 *)g_ps.lev[L].par.p,
-              (u32 *)g_ps.lev[L].mv.p);
+              (const u64 *)g_ps.lev[L].off.p - i0, (u64)off[a0], (u64 *)g_ps.lev[L].pm.p);
       if (L + 1 == D - 1) {
-        ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u32 *)g_ps.lev[L].par.p,
-                (const u32 *)g_ps.lev[L].mv.p, (size_t)m, d_counts, nobulk);
+        ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u64 *)g_ps.lev[L].pm.p,
+                (size_t)m, d_counts, nobulk);
       } else {
         if ((r = ensure_level(L + 1, m))) return r;
         ZLAUNCH(zr::K_PERFT_APPLY, m, k_apply, blocks(m, TPB), TPB, 0, front_of(L),
-                (const u32 *)g_ps.lev[L].par.p, (const u32 *)g_ps.lev[L].mv.p, (size_t)m, front_of(L + 1));
+                (const u64 *)g_ps.lev[L].pm.p, (size_t)m, front_of(L + 1));
         if ((r = process(L + 1, 0, m, D, d_counts, nobulk))) return r;
       }
     }
@@ -400,15 +400,15 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
   }
   if ((r = ensure_pairs(L, total))) return r;
   ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, TPB), TPB, 0, front_of(L), i0, i1,
-          (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u32 *)g_ps.lev[L].par.p, (u32 *)g_ps.lev[L].mv.p);
+          (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p);
   if (L + 1 == D - 1) {
     ZLAUNCH(zr::K_PERFT_LEAF, total, k_leaf, blocks(total, TPB), TPB, 0, front_of(L),
-            (const u32 *)g_ps.lev[L].par.p, (const u32 *)g_ps.lev[L].mv.p, (size_t)total, d_counts, nobulk);
+            (const u64 *)g_ps.lev[L].pm.p, (size_t)total, d_counts, nobulk);
     return ZETA_OK;
   }
   if ((r = ensure_level(L + 1, total))) return r;
   ZLAUNCH(zr::K_PERFT_APPLY, total, k_apply, blocks(total, TPB), TPB, 0, front_of(L),
-          (const u32 *)g_ps.lev[L].par.p, (const u32 *)g_ps.lev[L].mv.p, (size_t)total, front_of(L + 1));
+          (const u64 *)g_ps.lev[L].pm.p, (size_t)total, front_of(L + 1));
   return process(L + 1, 0, total, D, d_counts, nobulk);
 }
 
@@ -441,7 +441,7 @@ void perft_release() {
   for (auto &l : g_ps.lev) {
     zr::release(l.b0); zr::release(l.b1); zr::release(l.b2); zr::release(l.b3);
     zr::release(l.cr); zr::release(l.meta); zr::release(l.cnt); zr::release(l.off);
-    zr::release(l.bsum); zr::release(l.par); zr::release(l.mv);
+    zr::release(l.bsum); zr::release(l.pm);
     l.cap = l.paircap = 0;
   }
   zr::release(g_ps.total);

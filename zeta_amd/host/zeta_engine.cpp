@@ -408,6 +408,26 @@ void think() {
   }
 }
 
+// one fixed-depth search of a position with the configured workers, the
+// reference's benchmark() (src/bench.c:36-112); returns the seconds taken
+double benchmark(const uint64_t *board, bool stm, int32_t depth) {
+  ABNODECOUNT = 0;
+  TTHITS = 0;
+  memcpy(GLOBAL_BOARD, board, 7 * sizeof(uint64_t));
+  fill_history();
+  const double start = now_ms();
+  if (!cl_write_objects() || !cl_run_alphabeta(stm, depth, MaxNodes) || !cl_read_memory()) return -1;
+  const double el = (now_ms() - start + 1) / 1000;
+  for (uint64_t i = 0; i < totalWorkUnits; i++) {
+    ABNODECOUNT += COUNTERS[i * 64 + 1];
+    TTHITS += COUNTERS[i * 64 + 3];
+  }
+  const int32_t sc = (int32_t)PV[0];
+  out("depth: %i, nodes %" PRIu64 ", nps: %i, time: %lf sec, score: %i  move %s\n", depth, ABNODECOUNT,
+      (int)((double)ABNODECOUNT / el), el, (sc == INF || sc == -INF) ? 0 : sc, move2can(PV[1]).c_str());
+  return el;
+}
+
 // ---------------------------------------------------------------- tests
 // the reference's known-answer perft positions (src/test.c:43-73)
 struct Kat {
@@ -676,6 +696,27 @@ void xboard() {
       out("nodecount:%" PRIu64 ", seconds: %lf, nps: %" PRIu64 " \n", ABNODECOUNT, el,
           (uint64_t)((double)ABNODECOUNT / el));
       if (!cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+    } else if (!xboard_mode && !strcmp(cmd, "benchsmp")) {
+      // time to depth SD of the current position for 1, 2, 4, .. workers per
+      // root up to the configured count (src/xboard.c:822-974)
+      const uint64_t wmax = totalWorkUnits;
+      std::vector<double> secs;
+      std::vector<uint64_t> ws;
+      for (uint64_t w = 1;; w *= 2) {
+        if (w > wmax) w = wmax;
+        totalWorkUnits = w;
+        if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+        const double el = benchmark(BOARD, STM, SD);
+        if (el < 0) quitengine(EXIT_FAILURE);
+        secs.push_back(el);
+        ws.push_back(w);
+        if (w == wmax) break;
+      }
+      totalWorkUnits = wmax;
+      if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+      out("#\n# overall results\n### workers\t#ttd speedup\t#rel ttd speedup ###\n");
+      for (size_t i = 0; i < secs.size(); i++)
+        out("### %" PRIu64 "\t\t%lf\t%lf \n", ws[i], secs[0] / secs[i], i ? secs[i - 1] / secs[i] : 1.0);
     } else if (!xboard_mode && !strcmp(cmd, "bench")) {
       // fixed-depth search of the current position (time to depth and nps)
       const double start = now_ms();
