@@ -32,13 +32,7 @@
 
 namespace zc {
 
-#ifdef ZC_U64
-typedef ZC_U64 u64;  // op-counting host build (tests/hostcore/opcount.cpp)
-#define ZC_RAW(x) zc_raw(x)
-#else
 typedef uint64_t u64;
-#define ZC_RAW(x) (x)
-#endif
 typedef uint32_t u32;
 typedef int32_t s32;
 typedef uint16_t u16;
@@ -60,9 +54,9 @@ constexpr u64 SMCRBLACKK = 0x9000000000000000ULL;
 
 enum { PNONE = 0, PAWN = 1, KNIGHT = 2, KING = 3, BISHOP = 4, ROOK = 5, QUEEN = 6 };
 
-ZD int popc(u64 x) { return __builtin_popcountll(ZC_RAW(x)); }
-ZD int lsb(u64 x) { return __builtin_ctzll(ZC_RAW(x)); }
-ZD int msb(u64 x) { return 63 - __builtin_clzll(ZC_RAW(x)); }
+ZD int popc(u64 x) { return __builtin_popcountll(x); }
+ZD int lsb(u64 x) { return __builtin_ctzll(x); }
+ZD int msb(u64 x) { return 63 - __builtin_clzll(x); }
 // Q4: shift counts are taken modulo 64, as OpenCL C does.
 ZD u64 bit(int sq) { return u64(1) << (sq & 63); }
 ZD u64 shlm(u64 x, int n) { return x << (n & 63); }
@@ -183,9 +177,9 @@ ZD u64 anti_line(int k) {  // h1-a8 direction
 // per lane: the search analyses one position with 64 lanes, lane = square.
 ZD u64 rev64(u64 x) {
 #if defined(__clang__)
-  return __builtin_bitreverse64(ZC_RAW(x));
+  return __builtin_bitreverse64(x);
 #else
-  u64 r = ZC_RAW(x);
+  u64 r = x;
   r = ((r >> 1) & 0x5555555555555555ULL) | ((r & 0x5555555555555555ULL) << 1);
   r = ((r >> 2) & 0x3333333333333333ULL) | ((r & 0x3333333333333333ULL) << 2);
   r = ((r >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((r & 0x0F0F0F0F0F0F0F0FULL) << 4);
@@ -274,20 +268,6 @@ ZD void analyse(const Pos &p, Legal &L) {
   L.pinA = pa;
   const int n = popc(checkers);
   L.cmask = n == 0 ? FULL : (n == 1 ? (checkers | chkray) : 0ULL);
-}
-
-// is the side to move in check?  (cheap: attackers of the king square only)
-ZD bool in_check(const Pos &p) {
-  const u64 occ = p.b1 | p.b2 | p.b3;
-  const u64 own = p.stm ? p.b0 : (p.b0 ^ occ);
-  const u64 opp = occ ^ own;
-  const u64 kbb = own & p.b1 & p.b2 & ~p.b3;
-  const u64 e = ~occ;
-  const u64 rq = p.b3 & (p.b1 ^ p.b2), bq = p.b3 & ~p.b1;
-  u64 att = (rook_att(kbb, e) & opp & rq) | (bishop_att(kbb, e) & opp & bq);
-  att |= knight_att(kbb) & opp & (~p.b1 & p.b2 & ~p.b3);
-  att |= pawn_att(kbb, p.stm) & opp & (p.b1 & ~p.b2 & ~p.b3);
-  return att != 0;
 }
 
 // is the side to move's king attacked after the e.p. capture from->to taking sqep?

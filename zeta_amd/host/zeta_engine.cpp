@@ -147,7 +147,11 @@ std::vector<uint32_t> legal_moves(const uint64_t *b, bool stm) {
   return mv;
 }
 
-bool in_check(const uint64_t *b, bool stm) { return zc::in_check(to_pos(b, stm)); }
+bool in_check(const uint64_t *b, bool stm) {
+  zc::Node0 B;
+  zc::node_basics(to_pos(b, stm), B);
+  return B.checkers != 0;
+}
 
 // host domove (src/bitboard.c:583-693): planes, moved flags with the castle
 // rook, e.p. flag, half-move clock, hash from scratch
