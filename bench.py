@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the fixed-depth search leg")
     ap.add_argument("--search-depth", type=int, default=6)
-    ap.add_argument("--search-workers", type=int, default=64)
+    ap.add_argument("--search-workers", type=int, default=32)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
                     help="also run the library's streaming copy kernel once over BYTES (HBM counter calibration)")
     args = ap.parse_args()
