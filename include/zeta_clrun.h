@@ -23,9 +23,10 @@
  * nodes) searches with iterative deepening 1..d inside one call (earlier
  * iterations of the host's own loop hit the table exactly, so the repeated
  * work is small); the node budget `nodes` applies per worker, as in the
- * reference (src/zeta.cl:1945); positions before the root in
- * GLOBAL_HASHHISTORY are not consulted for repetitions (the search checks
- * repetitions inside the searched path only). */
+ * reference (src/zeta.cl:1945); of the positions before the root in
+ * GLOBAL_HASHHISTORY (worker 0's copy, entries 0..PLY-1) the newest
+ * ZETA_HISTORY (128) are consulted for repetitions, together with the
+ * searched path (src/zeta.cl:1560-1577). */
 #ifndef ZETA_CLRUN_H
 #define ZETA_CLRUN_H
 
