@@ -37,6 +37,16 @@ extern "C" int hc_enumerate(const uint64_t *b, int stm, int qs, uint32_t *moves)
   return n;
 }
 
+// the grouped enumeration the perft expansion kernel writes pairs with
+extern "C" int hc_enumerate_sets(const uint64_t *b, int stm, uint32_t *moves) {
+  Pos p = load(b, stm);
+  Legal L;
+  analyse(p, L);
+  int n = 0;
+  for_each_legal(p, L, [&](u32 code) { moves[n++] = code; });
+  return n;
+}
+
 extern "C" void hc_apply(const uint64_t *b, int stm, uint32_t m, int rook_flag, uint64_t *out) {
   Pos p = load(b, stm);
   Pos c = apply(p, m, rook_flag != 0);

@@ -24,6 +24,7 @@ def H():
     vp = ctypes.c_void_p
     L.hc_count.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_enumerate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.hc_enumerate_sets.argtypes = [vp, ctypes.c_int, vp]
     L.hc_perft.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_perft.restype = ctypes.c_uint64
     L.hc_enumerate_lane.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -79,6 +80,41 @@ def test_special_positions(H):
         b = np.ascontiguousarray(p["b"][0])
         for d in (1, 2, 3):
             assert H.hc_perft(b.ctypes.data, int(p["stm"][0]), d) == R.perft(p, d)
+
+
+SPECIAL_FENS = ["R6R/3Q4/1Q4Q1/4Q3/2Q4Q/Q4Q2/pp1Q4/kBNN1KB1 w - - 0 1",  # 218 moves
+                "r3k2r/8/8/8/8/8/8/R3K2R w KQkq - 0 1", "8/8/8/8/k2Pp2Q/8/8/3K4 b - - 0 1",
+                "8/8/8/2k5/3Pp3/8/8/4K2R b K - 0 1", "7k/5Q2/6K1/8/8/8/8/8 b - - 0 1",
+                "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1",
+                "8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - 0 1",
+                "n1n5/PPPk4/8/8/8/8/4Kppp/5N1N b - - 0 1"]
+
+
+def _grouped_vs_oracle(H, p, buf):
+    b = np.ascontiguousarray(p["b"][0])
+    n = H.hc_enumerate_sets(b.ctypes.data, int(p["stm"][0]), buf.ctypes.data)
+    om = _oracle_moves(p, 0)
+    assert n == len(om) == H.hc_count(b.ctypes.data, int(p["stm"][0]), 0)
+    assert sorted(buf[:n].tolist()) == sorted(om)  # same codes; the order is by kind
+
+
+def test_grouped_enumeration_matches_oracle(H):
+    # the perft expansion kernel's enumeration (by move kind, origin recovered
+    # from each target) gives exactly the oracle's legal moves: playouts, and
+    # every position within two plies of positions rich in castling, e.p.,
+    # promotions, pins and checks
+    buf = np.zeros(256, dtype=np.uint32)
+    pos = np.concatenate([R.gen_playouts(5, 1500, 8, 40), R.gen_playouts(6, 500, 20, 60)])
+    for i in range(len(pos)):
+        _grouped_vs_oracle(H, pos[i:i + 1], buf)
+    for fen in SPECIAL_FENS:
+        root = R.setboard(fen)
+        _grouped_vs_oracle(H, root, buf)
+        for m in _oracle_moves(root, 0):
+            c = R.domove(root, m)
+            _grouped_vs_oracle(H, c, buf)
+            for m2 in _oracle_moves(c, 0)[:40]:
+                _grouped_vs_oracle(H, R.domove(c, m2), buf)
 
 
 def test_single_square_sliders_equal_fills(H):

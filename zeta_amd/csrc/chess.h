@@ -63,6 +63,30 @@ ZD u64 shlm(u64 x, int n) { return x << (n & 63); }
 ZD u64 shrm(u64 x, int n) { return x >> (n & 63); }
 ZD u64 sel(bool c, u64 x) { return x & (0ULL - (u64)c); }
 
+// A 64-bit value the device compiler must keep whole in a register pair (an
+// empty asm statement): it then shifts it with one v_lshlrev_b64 instead of
+// splitting the shift into 32-bit halves for consumers that use halves.
+ZD u64 whole(u64 x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+v"(x));
+#endif
+  return x;
+}
+// (a & b) | c.  The compiler splits 64-bit AND and OR into 32-bit halves
+// before it looks for the fused v_and_or_b32, so plain C costs four VALU
+// instructions here; written out per half it is two.  Per-lane values only
+// (the operands are forced into vector registers).
+ZD u64 and_or(u64 a, u64 b, u64 c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  u32 lo, hi;
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(lo) : "v"((u32)a), "v"((u32)b), "v"((u32)c));
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(hi) : "v"((u32)(a >> 32)), "v"((u32)(b >> 32)), "v"((u32)(c >> 32)));
+  return whole(((u64)hi << 32) | lo);
+#else
+  return (a & b) | c;
+#endif
+}
+
 // ---- move code: the reference's 32-bit packing (src/types.h:67-74,126-141)
 ZD u32 mv_from(u32 m) { return m & 63; }
 ZD u32 mv_to(u32 m) { return (m >> 6) & 63; }
@@ -99,6 +123,15 @@ ZD u64 step(u64 x, int d) {
     default: return (x >> 9) & NOTH;
   }
 }
+// one doubling step of an occluded fill: g | (e & (g shifted by SH)).  A
+// shift by 32 or more moves one half into the other, which plain C already
+// does without instructions; shorter shifts use the fused AND-OR.
+template <int SH, bool UP>
+ZD u64 fill_step(u64 g, u64 e) {
+  const u64 t = UP ? g << SH : g >> SH;
+  if (SH >= 32) return g | (e & t);
+  return and_or(e, whole(t), g);
+}
 // occluded fill: generators plus every square reachable through `empty`
 template <int D>
 ZD u64 ks_fill(u64 g, u64 e) {
@@ -107,24 +140,37 @@ ZD u64 ks_fill(u64 g, u64 e) {
   constexpr u64 m = (D == 2 || D == 4 || D == 6) ? NOTA : (D == 3 || D == 5 || D == 7) ? NOTH : FULL;
   e &= m;
   if (up) {
-    g |= e & (g << s);
+    g = fill_step<s, true>(g, e);
     e &= e << s;
-    g |= e & (g << (2 * s));
+    g = fill_step<2 * s, true>(g, e);
     e &= e << (2 * s);
-    g |= e & (g << (4 * s));
+    g = fill_step<4 * s, true>(g, e);
   } else {
-    g |= e & (g >> s);
+    g = fill_step<s, false>(g, e);
     e &= e >> s;
-    g |= e & (g >> (2 * s));
+    g = fill_step<2 * s, false>(g, e);
     e &= e >> (2 * s);
-    g |= e & (g >> (4 * s));
+    g = fill_step<4 * s, false>(g, e);
   }
   return g;
+}
+// the one-step shift of direction D without its edge-file mask (NOTA for
+// E, NE, SE; NOTH for W, NW, SW; none for N, S): directions that share a
+// mask can be masked once, after their union or on the target set
+template <int D>
+ZD u64 shift_raw(u64 x) {
+  return D == 0 ? x << 8 : D == 1 ? x >> 8 : D == 2 ? x << 1 : D == 3 ? x >> 1
+       : D == 4 ? x << 9 : D == 5 ? x << 7 : D == 6 ? x >> 7 : x >> 9;
 }
 // squares a slider set attacks in direction D (through empties, incl. first blocker)
 template <int D>
 ZD u64 slide(u64 g, u64 empty) {
   return step(ks_fill<D>(g, empty), D);
+}
+// the same without the edge-file mask
+template <int D>
+ZD u64 slide_raw(u64 g, u64 empty) {
+  return shift_raw<D>(ks_fill<D>(g, empty));
 }
 ZD u64 rook_att(u64 g, u64 empty) {
   return slide<0>(g, empty) | slide<1>(g, empty) | slide<2>(g, empty) | slide<3>(g, empty);
@@ -137,19 +183,6 @@ ZD u64 knight_att(u64 n) {
   u64 r1 = (n << 1) & NOTA, r2 = (n << 2) & 0xFCFCFCFCFCFCFCFCULL;
   u64 h1 = l1 | r1, h2 = l2 | r2;
   return (h1 << 16) | (h1 >> 16) | (h2 << 8) | (h2 >> 8);
-}
-// the 8 knight offsets separately (each shift is injective on the board)
-ZD u64 knight_dir(u64 n, int k) {
-  switch (k) {
-    case 0: return (n << 17) & NOTA;
-    case 1: return (n << 15) & NOTH;
-    case 2: return (n << 10) & 0xFCFCFCFCFCFCFCFCULL;
-    case 3: return (n << 6) & 0x3F3F3F3F3F3F3F3FULL;
-    case 4: return (n >> 6) & 0xFCFCFCFCFCFCFCFCULL;
-    case 5: return (n >> 10) & 0x3F3F3F3F3F3F3F3FULL;
-    case 6: return (n >> 15) & NOTA;
-    default: return (n >> 17) & NOTH;
-  }
 }
 ZD u64 king_att(u64 k) {
   u64 a = k | ((k << 1) & NOTA) | ((k >> 1) & NOTH);
@@ -218,14 +251,18 @@ struct Legal {
 // Checkers, check ray and pins along one line through the king (m = the line
 // without the king square, sl = the opponent sliders that move along it):
 // single-square attacks from the king give the first piece each way; with
-// the own pieces among them removed, the next pieces show the pinners.
-ZD void king_line(const Legal &L, u64 m, u64 sl, u64 &checkers, u64 &chkray, u64 &pin) {
+// the own pieces among them removed, the next pieces show the pinners.  A
+// checker's attack continues through the king as far as the king's own
+// attack on the other side reaches: `xray` collects those squares.
+ZD void king_line(const Legal &L, u64 m, u64 sl, u64 &checkers, u64 &chkray, u64 &xray, u64 &pin) {
   const int k = L.ksq;
   const u64 lo = m & (bit(k) - 1), up = m & ~lo;
   const u64 att = line_att(k, L.occ, m);
   const u64 chk = att & sl;
+  const bool cu = (chk & up) != 0, cl = (chk & lo) != 0;
   checkers |= chk;
-  chkray |= sel((chk & up) != 0, att & up) | sel((chk & lo) != 0, att & lo);
+  chkray |= sel(cu, att & up) | sel(cl, att & lo);
+  xray |= sel(cu, att & lo) | sel(cl, att & up);
   const u64 blk = att & L.own;
   const u64 pn = line_att(k, L.occ ^ blk, m) & ~att & sl;
   pin |= sel((pn & up) != 0, blk & up) | sel((pn & lo) != 0, blk & lo);
@@ -246,21 +283,25 @@ ZD void analyse(const Pos &p, Legal &L) {
   L.kbb = L.own & L.kings;
   L.ksq = L.kbb ? lsb(L.kbb) : 0;
   const u64 orq = L.opp & L.rq, obq = L.opp & L.bq;
-  const u64 enk = ~(occ ^ L.kbb);
+  // the opponent's attacks with the own king as a blocker (the fills share
+  // their empty set with the own move counts), completed below by the
+  // squares the checking sliders attack through the king
+  const u64 e = ~occ;
   u64 A = pawn_att(L.opp & L.pawns, p.stm ^ 1u) | knight_att(L.opp & L.knights) | king_att(L.opp & L.kings);
-  A |= slide<0>(orq, enk) | slide<1>(orq, enk) | slide<2>(orq, enk) | slide<3>(orq, enk);
-  A |= slide<4>(obq, enk) | slide<5>(obq, enk) | slide<6>(obq, enk) | slide<7>(obq, enk);
-  L.A = A;
+  A |= slide_raw<0>(orq, e) | slide_raw<1>(orq, e);
+  A |= (slide_raw<2>(orq, e) | slide_raw<4>(obq, e) | slide_raw<6>(obq, e)) & NOTA;
+  A |= (slide_raw<3>(orq, e) | slide_raw<5>(obq, e) | slide_raw<7>(obq, e)) & NOTH;
   u64 checkers = (pawn_att(L.kbb, p.stm) & L.opp & L.pawns) | (knight_att(L.kbb) & L.opp & L.knights);
-  u64 chkray = 0, pr = 0, pf = 0, pd = 0, pa = 0;
+  u64 chkray = 0, xray = 0, pr = 0, pf = 0, pd = 0, pa = 0;
   if (L.kbb) {
     const int k = L.ksq;
     const u64 kb = bit(k);
-    king_line(L, file_line(k) ^ kb, orq, checkers, chkray, pf);
-    king_line(L, rank_line(k) ^ kb, orq, checkers, chkray, pr);
-    king_line(L, diag_line(k) ^ kb, obq, checkers, chkray, pd);
-    king_line(L, anti_line(k) ^ kb, obq, checkers, chkray, pa);
+    king_line(L, file_line(k) ^ kb, orq, checkers, chkray, xray, pf);
+    king_line(L, rank_line(k) ^ kb, orq, checkers, chkray, xray, pr);
+    king_line(L, diag_line(k) ^ kb, obq, checkers, chkray, xray, pd);
+    king_line(L, anti_line(k) ^ kb, obq, checkers, chkray, xray, pa);
   }
+  L.A = A | xray;
   L.checkers = checkers;
   L.pinR = pr;
   L.pinF = pf;
@@ -313,21 +354,26 @@ ZD u64 castle_targets(const Pos &p, const Legal &L) {
 ZD int count_legal(const Pos &p, const Legal &L, bool qs) {
   const u64 own = L.own, occ = L.occ;
   const u64 T = (qs ? L.opp : ~own) & L.cmask;
+  // the target set with each edge-file mask applied once (a shifted set
+  // masked and then intersected with T is the shifted set intersected with
+  // the masked T)
+  const u64 TA = T & NOTA, TH = T & NOTH;
   const u64 pinned = L.pinR | L.pinF | L.pinD | L.pinA;
   int cnt = popc(king_att(L.kbb) & (qs ? L.opp : ~own) & ~L.A);
   const u64 kn = own & L.knights & ~pinned;
-#pragma unroll
-  for (int k = 0; k < 8; k++) cnt += popc(knight_dir(kn, k) & T);
+  const u64 TAB = T & 0xFCFCFCFCFCFCFCFCULL, TGH = T & 0x3F3F3F3F3F3F3F3FULL;
+  cnt += popc((kn << 17) & TA) + popc((kn << 15) & TH) + popc((kn << 10) & TAB) + popc((kn << 6) & TGH);
+  cnt += popc((kn >> 6) & TAB) + popc((kn >> 10) & TGH) + popc((kn >> 15) & TA) + popc((kn >> 17) & TH);
   const u64 e = ~occ;
   const u64 orq = own & L.rq, obq = own & L.bq;
-  cnt += popc(slide<0>(orq & (~pinned | L.pinF), e) & T);
-  cnt += popc(slide<1>(orq & (~pinned | L.pinF), e) & T);
-  cnt += popc(slide<2>(orq & (~pinned | L.pinR), e) & T);
-  cnt += popc(slide<3>(orq & (~pinned | L.pinR), e) & T);
-  cnt += popc(slide<4>(obq & (~pinned | L.pinD), e) & T);
-  cnt += popc(slide<7>(obq & (~pinned | L.pinD), e) & T);
-  cnt += popc(slide<5>(obq & (~pinned | L.pinA), e) & T);
-  cnt += popc(slide<6>(obq & (~pinned | L.pinA), e) & T);
+  cnt += popc(slide_raw<0>(orq & (~pinned | L.pinF), e) & T);
+  cnt += popc(slide_raw<1>(orq & (~pinned | L.pinF), e) & T);
+  cnt += popc(slide_raw<2>(orq & (~pinned | L.pinR), e) & TA);
+  cnt += popc(slide_raw<3>(orq & (~pinned | L.pinR), e) & TH);
+  cnt += popc(slide_raw<4>(obq & (~pinned | L.pinD), e) & TA);
+  cnt += popc(slide_raw<7>(obq & (~pinned | L.pinD), e) & TH);
+  cnt += popc(slide_raw<5>(obq & (~pinned | L.pinA), e) & TH);
+  cnt += popc(slide_raw<6>(obq & (~pinned | L.pinA), e) & TA);
   const u64 P = own & L.pawns;
   const u64 capD = P & (~pinned | L.pinD), capA = P & (~pinned | L.pinA);
   const u64 tgt = L.opp & L.cmask;
@@ -677,6 +723,128 @@ ZD u32 nth_legal(const Pos &p, const Legal &L, int k) {
     k -= c;
   }
   return 0;
+}
+
+// Origin of a slider move to `to` in direction d: the path from the origin is
+// empty, so it is the nearest occupied square back along the direction.
+ZD int slider_from(int d, int to, u64 occ) {
+  const u64 b = bit(to), below = b - 1, above = ~(below | b);
+  switch (d) {
+    case 0: return msb(occ & file_line(to) & below);  // moved north
+    case 1: return lsb(occ & file_line(to) & above);  // south
+    case 2: return msb(occ & rank_line(to) & below);  // east
+    case 3: return lsb(occ & rank_line(to) & above);  // west
+    case 4: return msb(occ & diag_line(to) & below);  // north-east
+    case 5: return msb(occ & anti_line(to) & below);  // north-west
+    case 6: return lsb(occ & anti_line(to) & above);  // south-east
+    default: return lsb(occ & diag_line(to) & above);  // south-west
+  }
+}
+
+// Every legal move (the set count_legal(qs = false) counts, so exactly that
+// many calls of emit(code)), grouped by kind instead of by origin: each
+// kind's target set is built for all its pieces at once, as in count_legal,
+// and each target gives back its origin -- a fixed offset for king, knight and
+// pawn moves, slider_from() for sliders.  Codes equal make_code(p, from, to);
+// the order is not the canonical one (nth_legal).
+template <class Emit>
+ZD void for_each_legal(const Pos &p, const Legal &L, Emit &&emit) {
+  const u32 stm = p.stm;
+  const u64 own = L.own, occ = L.occ, e = ~occ;
+  const u64 T = ~own & L.cmask;
+  const u64 TA = T & NOTA, TH = T & NOTH;
+  const u64 pinned = L.pinR | L.pinF | L.pinD | L.pinA;
+  // one kind with a fixed origin offset (from = to - delta) and moving piece pf
+  auto fixed = [&](u64 t, int delta, u32 pf) {
+    while (t) {
+      const int to = lsb(t);
+      t &= t - 1;
+      emit(mv_make((u32)(to - delta), (u32)to, (u32)to, pf, pf, piece_at(p, to)));
+    }
+  };
+  // king (castling included: the rook's move is implied by the code)
+  {
+    const u32 pk = (KING << 1) | stm;
+    u64 t = (king_att(L.kbb) & ~own & ~L.A) | castle_targets(p, L);
+    while (t) {
+      const int to = lsb(t);
+      t &= t - 1;
+      emit(mv_make((u32)L.ksq, (u32)to, (u32)to, pk, pk, piece_at(p, to)));
+    }
+  }
+  // knights, one offset at a time
+  {
+    const u64 kn = own & L.knights & ~pinned;
+    const u64 TAB = T & 0xFCFCFCFCFCFCFCFCULL, TGH = T & 0x3F3F3F3F3F3F3F3FULL;
+    const u32 pn = (KNIGHT << 1) | stm;
+    fixed((kn << 17) & TA, 17, pn);
+    fixed((kn << 15) & TH, 15, pn);
+    fixed((kn << 10) & TAB, 10, pn);
+    fixed((kn << 6) & TGH, 6, pn);
+    fixed((kn >> 6) & TAB, -6, pn);
+    fixed((kn >> 10) & TGH, -10, pn);
+    fixed((kn >> 15) & TA, -15, pn);
+    fixed((kn >> 17) & TH, -17, pn);
+  }
+  // sliders, one direction at a time
+  {
+    const u64 orq = own & L.rq, obq = own & L.bq;
+    auto dir = [&](int d, u64 t) {
+      while (t) {
+        const int to = lsb(t);
+        t &= t - 1;
+        const int from = slider_from(d, to, occ);
+        const u32 pf = piece_at(p, from);
+        emit(mv_make((u32)from, (u32)to, (u32)to, pf, pf, piece_at(p, to)));
+      }
+    };
+    dir(0, slide_raw<0>(orq & (~pinned | L.pinF), e) & T);
+    dir(1, slide_raw<1>(orq & (~pinned | L.pinF), e) & T);
+    dir(2, slide_raw<2>(orq & (~pinned | L.pinR), e) & TA);
+    dir(3, slide_raw<3>(orq & (~pinned | L.pinR), e) & TH);
+    dir(4, slide_raw<4>(obq & (~pinned | L.pinD), e) & TA);
+    dir(7, slide_raw<7>(obq & (~pinned | L.pinD), e) & TH);
+    dir(5, slide_raw<5>(obq & (~pinned | L.pinA), e) & TH);
+    dir(6, slide_raw<6>(obq & (~pinned | L.pinA), e) & TA);
+  }
+  // pawns: pushes, double pushes, captures (queen-only promotion, Q1), e.p.
+  {
+    const u64 P = own & L.pawns;
+    const u32 pp_ = (PAWN << 1) | stm, pq = (QUEEN << 1) | stm;
+    const int last = stm ? 0 : 7;
+    auto pawn = [&](u64 t, int delta) {
+      while (t) {
+        const int to = lsb(t);
+        t &= t - 1;
+        const u32 pt = (to >> 3) == last ? pq : pp_;
+        emit(mv_make((u32)(to - delta), (u32)to, (u32)to, pp_, pt, piece_at(p, to)));
+      }
+    };
+    const u64 pp = P & (~pinned | L.pinF);
+    const u64 capD = P & (~pinned | L.pinD), capA = P & (~pinned | L.pinA);
+    const u64 tgt = L.opp & L.cmask;
+    if (stm) {
+      const u64 s1 = (pp >> 8) & e, s2 = ((s1 & RANK6) >> 8) & e;
+      pawn(s1 & L.cmask, -8);
+      pawn(s2 & L.cmask, -16);
+      pawn(((capD >> 9) & NOTH) & tgt, -9);
+      pawn(((capA >> 7) & NOTA) & tgt, -7);
+    } else {
+      const u64 s1 = (pp << 8) & e, s2 = ((s1 & RANK3) << 8) & e;
+      pawn(s1 & L.cmask, 8);
+      pawn(s2 & L.cmask, 16);
+      pawn(((capD << 9) & NOTA) & tgt, 9);
+      pawn(((capA << 7) & NOTH) & tgt, 7);
+    }
+    int sqep;
+    u64 cand = ep_candidates(p, L, sqep);
+    while (cand) {
+      const int f = lsb(cand);
+      cand &= cand - 1;
+      const int to = (stm ? sqep - 8 : sqep + 8) & 63;
+      if (ep_legal(L, f, to, sqep)) emit(make_code(p, f, to));
+    }
+  }
 }
 
 // host domove half-move clock rule (src/bitboard.c:604-689): +1, reset by

@@ -139,26 +139,47 @@ __global__ __launch_bounds__(TPB) void k_scan_down(const u32 *__restrict__ cnt, 
   if (base < n && base + SCAN_ITEMS >= n) off[n] = ex;
 }
 
-// ---- expand: (parent, move) pairs of every legal move, at the scanned offsets
-__global__ __launch_bounds__(TPB) void k_expand(Front f, size_t i0, size_t i1, const u64 *__restrict__ off,
-                                                u64 base, u64 *__restrict__ pm) {
-  const size_t i = i0 + (size_t)blockIdx.x * TPB + threadIdx.x;
-  if (i >= i1) return;
-  const Pos p = load_pos(f, i);
-  Legal L;
-  analyse(p, L);
-  u64 o = off[i] - base;
-  u64 pcs = L.own;
-  while (pcs) {
-    const int sq = lsb(pcs);
-    pcs &= pcs - 1;
-    u64 t = piece_targets(p, L, sq, false);
-    while (t) {
-      const int to = lsb(t);
-      t &= t - 1;
-      pm[o] = ((u64)i << 32) | make_code(p, sq, to);  // parent index : move, one 8-byte store
+// ---- expand: (parent, move) pairs of every legal move, at the scanned offsets.
+// A block's parents own one contiguous range of pairs.  Each lane enumerates
+// its parent's moves into LDS (move code + the lane that owns the parent),
+// then the block writes the range out with coalesced 8-byte stores: stored
+// straight from the enumeration loop, the pairs of 64 lanes land on 64
+// different cache lines per store instruction.  Moves beyond the LDS capacity
+// (a block whose parents have more than XCAP moves in all) are stored directly.
+constexpr int XTPB = 128;
+constexpr int XCAP = 4096;  // 16 KiB of codes + 4 KiB of lane indices per block
+__global__ __launch_bounds__(XTPB) void k_expand(Front f, size_t i0, size_t i1, const u64 *__restrict__ off,
+                                                 u64 base, u64 *__restrict__ pm) {
+  __shared__ u32 lcode[XCAP];
+  __shared__ u8 llane[XCAP];
+  const size_t bi0 = i0 + (size_t)blockIdx.x * XTPB;
+  const size_t bi1 = bi0 + XTPB < i1 ? bi0 + XTPB : i1;
+  const u64 blo = off[bi0], total = off[bi1] - blo;
+  u64 *const out = pm + (blo - base);
+  const size_t i = bi0 + threadIdx.x;
+  if (i < bi1) {
+    const Pos p = load_pos(f, i);
+    Legal L;
+    analyse(p, L);
+    u64 o = off[i] - blo;
+    const u64 parent = (u64)i << 32;
+    const u8 lane = (u8)threadIdx.x;
+    for_each_legal(p, L, [&](u32 code) {
+      if (o < (u64)XCAP) {
+        lcode[o] = code;
+        llane[o] = lane;
+      } else {
+        out[o] = parent | code;
+      }
       o++;
-    }
+    });
+  }
+  __syncthreads();
+  const u32 n = total < (u64)XCAP ? (u32)total : (u32)XCAP;
+  const u32 last = (u32)(bi1 - bi0) - 1;  // every slot below n has an owner; the clamp only bounds the index
+  for (u32 j = threadIdx.x; j < n; j += XTPB) {
+    const u32 l = llane[j] < last ? llane[j] : last;
+    out[j] = ((u64)(bi0 + l) << 32) | lcode[j];
   }
 }
 
@@ -384,7 +405,7 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
       const size_t a0 = cuts[c], a1 = cuts[c + 1];
       const u64 m = off[a1] - off[a0];
       if ((r = ensure_pairs(L, m))) return r;
-      ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, TPB), TPB, 0, front_of(L), i0 + a0, i0 + a1,
+      ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, XTPB), XTPB, 0, front_of(L), i0 + a0, i0 + a1,
               (const u64 *)g_ps.lev[L].off.p - i0, (u64)off[a0], (u64 *)g_ps.lev[L].pm.p);
       if (L + 1 == D - 1) {
         ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u64 *)g_ps.lev[L].pm.p,
@@ -399,7 +420,7 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
     return ZETA_OK;
   }
   if ((r = ensure_pairs(L, total))) return r;
-  ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, TPB), TPB, 0, front_of(L), i0, i1,
+  ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, XTPB), XTPB, 0, front_of(L), i0, i1,
           (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p);
   if (L + 1 == D - 1) {
     ZLAUNCH(zr::K_PERFT_LEAF, total, k_leaf, blocks(total, TPB), TPB, 0, front_of(L),
