@@ -248,24 +248,26 @@ struct Legal {
   u32 stm;
 };
 
-// Checkers, check ray and pins along one line through the king (m = the line
-// without the king square, sl = the opponent sliders that move along it):
-// single-square attacks from the king give the first piece each way; with
-// the own pieces among them removed, the next pieces show the pinners.  A
-// checker's attack continues through the king as far as the king's own
-// attack on the other side reaches: `xray` collects those squares.
-ZD void king_line(const Legal &L, u64 m, u64 sl, u64 &checkers, u64 &chkray, u64 &xray, u64 &pin) {
-  const int k = L.ksq;
-  const u64 lo = m & (bit(k) - 1), up = m & ~lo;
-  const u64 att = line_att(k, L.occ, m);
-  const u64 chk = att & sl;
-  const bool cu = (chk & up) != 0, cl = (chk & lo) != 0;
-  checkers |= chk;
-  chkray |= sel(cu, att & up) | sel(cl, att & lo);
-  xray |= sel(cu, att & lo) | sel(cl, att & up);
-  const u64 blk = att & L.own;
-  const u64 pn = line_att(k, L.occ ^ blk, m) & ~att & sl;
-  pin |= sel((pn & up) != 0, blk & up) | sel((pn & lo) != 0, blk & lo);
+// Checkers, check ray, x-ray and pins along one line through the king, from
+// occluded fills in its two directions U and V: R = the rays of the opponent
+// sliders S that move along the line, K = the king's own rays (each up to and
+// including the first blocker).  The first piece on the king's U ray is a
+// checker if it is in S; an own piece there is pinned if an S ray in
+// direction V reaches it from the far side.  A checker's attack continues
+// through the king along the king's opposite ray (`xray`: squares the king
+// cannot step to).  Returns the pinned pieces; adds the rays to the attack
+// map A.
+template <int U, int V>
+ZD u64 king_line(u64 S, u64 kbb, u64 own, u64 e, u64 &A, u64 &checkers, u64 &chkray, u64 &xray) {
+  const u64 Ru = slide<U>(S, e), Rv = slide<V>(S, e);
+  const u64 Ku = slide<U>(kbb, e), Kv = slide<V>(kbb, e);
+  A |= Ru | Rv;
+  const u64 cu = Ku & S, cv = Kv & S;
+  checkers |= cu | cv;
+  const bool hu = cu != 0, hv = cv != 0;
+  chkray |= sel(hu, Ku) | sel(hv, Kv);
+  xray |= sel(hu, Kv) | sel(hv, Ku);
+  return ((Ku & Rv) | (Kv & Ru)) & own;
 }
 
 ZD void analyse(const Pos &p, Legal &L) {
@@ -283,30 +285,19 @@ ZD void analyse(const Pos &p, Legal &L) {
   L.kbb = L.own & L.kings;
   L.ksq = L.kbb ? lsb(L.kbb) : 0;
   const u64 orq = L.opp & L.rq, obq = L.opp & L.bq;
-  // the opponent's attacks with the own king as a blocker (the fills share
-  // their empty set with the own move counts), completed below by the
-  // squares the checking sliders attack through the king
+  // the opponent's attacks with the own king as a blocker (every fill shares
+  // the empty set, and with it the doubling masks, with the move counts),
+  // completed by the squares the checking sliders attack through the king
   const u64 e = ~occ;
   u64 A = pawn_att(L.opp & L.pawns, p.stm ^ 1u) | knight_att(L.opp & L.knights) | king_att(L.opp & L.kings);
-  A |= slide_raw<0>(orq, e) | slide_raw<1>(orq, e);
-  A |= (slide_raw<2>(orq, e) | slide_raw<4>(obq, e) | slide_raw<6>(obq, e)) & NOTA;
-  A |= (slide_raw<3>(orq, e) | slide_raw<5>(obq, e) | slide_raw<7>(obq, e)) & NOTH;
   u64 checkers = (pawn_att(L.kbb, p.stm) & L.opp & L.pawns) | (knight_att(L.kbb) & L.opp & L.knights);
-  u64 chkray = 0, xray = 0, pr = 0, pf = 0, pd = 0, pa = 0;
-  if (L.kbb) {
-    const int k = L.ksq;
-    const u64 kb = bit(k);
-    king_line(L, file_line(k) ^ kb, orq, checkers, chkray, xray, pf);
-    king_line(L, rank_line(k) ^ kb, orq, checkers, chkray, xray, pr);
-    king_line(L, diag_line(k) ^ kb, obq, checkers, chkray, xray, pd);
-    king_line(L, anti_line(k) ^ kb, obq, checkers, chkray, xray, pa);
-  }
+  u64 chkray = 0, xray = 0;
+  L.pinF = king_line<0, 1>(orq, L.kbb, L.own, e, A, checkers, chkray, xray);
+  L.pinR = king_line<2, 3>(orq, L.kbb, L.own, e, A, checkers, chkray, xray);
+  L.pinD = king_line<4, 7>(obq, L.kbb, L.own, e, A, checkers, chkray, xray);
+  L.pinA = king_line<5, 6>(obq, L.kbb, L.own, e, A, checkers, chkray, xray);
   L.A = A | xray;
   L.checkers = checkers;
-  L.pinR = pr;
-  L.pinF = pf;
-  L.pinD = pd;
-  L.pinA = pa;
   const int n = popc(checkers);
   L.cmask = n == 0 ? FULL : (n == 1 ? (checkers | chkray) : 0ULL);
 }
