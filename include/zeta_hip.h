@@ -49,7 +49,7 @@ typedef struct {
 typedef struct {
   int device;               /* HIP device ordinal */
   int tt_log2;              /* search TT entries = 2^tt_log2 (16 B each); 0 = no TT */
-  uint64_t frontier_budget; /* perft: max child moves held per level (0 = default 1<<28) */
+  uint64_t frontier_budget; /* perft: max child moves held per level (0 = default 1<<28; at least 512) */
   int workers;              /* search: wavefront workers per position (ABDADA), 0 = default */
   int verbose;
 } zeta_config;

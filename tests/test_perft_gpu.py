@@ -76,3 +76,31 @@ def test_edge_cases(zeta, oracle):
     assert list(zeta.perft(p, 1)) == [20, 0, 0, 218]
     for d in (2, 3):
         assert list(zeta.perft(p, d)) == [oracle.perft(p[i:i + 1], d) for i in range(len(p))]
+
+
+def test_chunked_levels_small_budget(zeta, oracle):
+    # a frontier budget far below a level's size splits levels into chunks
+    # (cuts computed on the device); counts must not change
+    pos = oracle.gen_playouts(2, 64, 8, 40)
+    want3 = oracle.perft_batch(pos, 3)
+    zeta.shutdown()
+    try:
+        zeta.init(frontier_budget=4096)
+        got = zeta.perft(pos, 3)
+        print("chunked perft(3) total", int(got.sum()), flush=True)
+        assert (got == want3).all()
+        d_pos = zeta.DeviceBuffer(pos.nbytes)
+        d_cnt = zeta.DeviceBuffer(8 * len(pos))
+        d_pos.upload(pos)
+        zeta.perft_device(d_pos.ptr, len(pos), 2, d_cnt.ptr, nobulk=True)
+        got2 = d_cnt.download(np.zeros(len(pos), dtype=np.uint64))
+        d_pos.free()
+        d_cnt.free()
+        assert (got2 == oracle.perft_batch(pos, 2)).all()
+        zeta.shutdown()
+        zeta.init(frontier_budget=100)
+        with pytest.raises(zeta.ZetaError):
+            zeta.perft(pos, 3)
+    finally:
+        zeta.shutdown()
+        zeta.init()

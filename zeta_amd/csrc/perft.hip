@@ -183,6 +183,28 @@ __global__ __launch_bounds__(XTPB) void k_expand(Front f, size_t i0, size_t i1, 
   }
 }
 
+// ---- chunk cuts of an oversized level: bucket b holds the parents whose pairs
+// start in [b * bp, (b + 1) * bp), so a chunk has at most bp + 218 pairs (the
+// most legal moves of a position).  cut_idx[b] = first parent of bucket b
+// (relative to the level range), cut_off[b] = its offset; buckets past the
+// last parent get (n, total).  Every b in [0, nb] is written exactly once.
+__global__ __launch_bounds__(TPB) void k_cuts(const u64 *__restrict__ off, size_t n, u64 bp, u64 nb,
+                                              u64 *__restrict__ cut_idx, u64 *__restrict__ cut_off) {
+  const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  const u64 b = off[i] / bp;
+  const u64 first = i ? off[i - 1] / bp + 1 : 0;
+  for (u64 j = first; j <= b && j <= nb; j++) {
+    cut_idx[j] = i;
+    cut_off[j] = off[i];
+  }
+  if (i + 1 == n)
+    for (u64 j = b + 1; j <= nb; j++) {
+      cut_idx[j] = n;
+      cut_off[j] = off[n];
+    }
+}
+
 // ---- apply: materialise the next level and count its moves
 __global__ __launch_bounds__(TPB) void k_apply(Front f, const u64 *__restrict__ pm,
                                                size_t m, Front g) {
@@ -323,7 +345,7 @@ struct LevelBufs {
 
 struct PerftState {
   LevelBufs lev[MAXLEV];
-  zr::DevBuf total, roots, counts, fails;
+  zr::DevBuf total, roots, counts, fails, cuts;
 };
 PerftState g_ps;
 
@@ -383,30 +405,30 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
   if (r) return r;
   if (total == 0) return ZETA_OK;
   const u64 budget = zr::ctx().cfg.frontier_budget;
-  // chunk boundaries (relative to i0) so that each chunk's pairs fit the budget
-  std::vector<size_t> cuts;
-  cuts.push_back(0);
+  // a level with more pairs than the budget runs in chunks of parents, each
+  // with at most `budget` pairs; the cuts come from the device (k_cuts) so the
+  // host reads a few words, not the level's offsets
   if (total > budget) {
-    std::vector<u64> off(i1 - i0 + 1);
-    ZCHECK(hipMemcpyAsync(off.data(), g_ps.lev[L].off.p, off.size() * 8, hipMemcpyDeviceToHost,
-                          zr::ctx().stream));
+    if (budget < 512) return zr::fail(ZETA_EINVAL, "frontier budget %llu is below 512 moves", (unsigned long long)budget);
+    const u64 bp = budget - 256;
+    const u64 nb = total / bp + 1;
+    const size_t n = i1 - i0;
+    if ((r = zr::ensure(g_ps.cuts, (size_t)(nb + 1) * 16))) return r;
+    u64 *d_idx = (u64 *)g_ps.cuts.p, *d_off = d_idx + nb + 1;
+    ZLAUNCH(zr::K_PERFT_SCAN, n, k_cuts, blocks(n, TPB), TPB, 0, (const u64 *)g_ps.lev[L].off.p, n, bp, nb, d_idx,
+            d_off);
+    std::vector<u64> cut(2 * (nb + 1));
+    ZCHECK(hipMemcpyAsync(cut.data(), d_idx, cut.size() * 8, hipMemcpyDeviceToHost, zr::ctx().stream));
     ZCHECK(hipStreamSynchronize(zr::ctx().stream));
-    size_t a = 0;
-    for (size_t k = 1; k <= i1 - i0; k++) {
-      if (off[k] - off[a] > budget) {
-        if (k - 1 == a) return zr::fail(ZETA_ENOMEM, "frontier budget smaller than one node's children");
-        cuts.push_back(k - 1);
-        a = k - 1;
-      }
-    }
-    cuts.push_back(i1 - i0);
-    // chunks re-read their own offsets, so keep the level's offset array intact
-    for (size_t c = 0; c + 1 < cuts.size(); c++) {
-      const size_t a0 = cuts[c], a1 = cuts[c + 1];
-      const u64 m = off[a1] - off[a0];
+    const u64 *cidx = cut.data(), *coff = cut.data() + nb + 1;
+    for (u64 c = 0; c < nb; c++) {
+      const size_t a0 = (size_t)cidx[c], a1 = (size_t)cidx[c + 1];
+      if (a1 <= a0) continue;
+      const u64 m = coff[c + 1] - coff[c];
+      if (m == 0) continue;
       if ((r = ensure_pairs(L, m))) return r;
       ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, XTPB), XTPB, 0, front_of(L), i0 + a0, i0 + a1,
-              (const u64 *)g_ps.lev[L].off.p - i0, (u64)off[a0], (u64 *)g_ps.lev[L].pm.p);
+              (const u64 *)g_ps.lev[L].off.p - i0, coff[c], (u64 *)g_ps.lev[L].pm.p);
       if (L + 1 == D - 1) {
         ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u64 *)g_ps.lev[L].pm.p,
                 (size_t)m, d_counts, nobulk);
@@ -469,6 +491,7 @@ void perft_release() {
   zr::release(g_ps.roots);
   zr::release(g_ps.counts);
   zr::release(g_ps.fails);
+  zr::release(g_ps.cuts);
 }
 
 extern "C" int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts, int flags) {
