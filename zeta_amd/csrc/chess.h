@@ -330,7 +330,18 @@ ZD u64 castle_targets(const Pos &p, const Legal &L) {
   if (!(cr & SMCRALL)) return 0;
   const bool okq = p.stm ? ((~cr & SMCRBLACKQ) == SMCRBLACKQ) : ((~cr & SMCRWHITEQ) == SMCRWHITEQ);
   const bool okk = p.stm ? ((~cr & SMCRBLACKK) == SMCRBLACKK) : ((~cr & SMCRWHITEK) == SMCRWHITEK);
+  if (!okq && !okk) return 0;
   const u64 ownrooks = L.own & L.rq & ~p.b2;  // rook = 101 (queen has b2 set)
+  if (k >= 4 && k <= 60) {
+    // the squares k-4..k+3 do not wrap: each group is one shifted constant
+    // (a king with a castle right is on e1/e8 unless the FEN was inconsistent)
+    const int q = k - 4;
+    u64 t = sel(okq && (ownrooks & shlm(0x1ULL, q)) && !(L.occ & shlm(0xEULL, q)) && !(L.A & shlm(0x1CULL, q)),
+                shlm(0x4ULL, q));
+    t |= sel(okk && (ownrooks & shlm(0x8ULL, k)) && !(L.occ & shlm(0x6ULL, k)) && !(L.A & shlm(0x7ULL, k)),
+             shlm(0x4ULL, k));
+    return t;
+  }
   u64 t = 0;
   t |= sel(okq && (ownrooks & bit(k - 4)) && !(L.occ & (bit(k - 1) | bit(k - 2) | bit(k - 3))) &&
                !(L.A & (bit(k) | bit(k - 1) | bit(k - 2))),
