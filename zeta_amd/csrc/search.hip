@@ -165,6 +165,8 @@ __device__ __forceinline__ u32 uf(u32 v) { return (u32)__builtin_amdgcn_readfirs
 __device__ __forceinline__ s32 uf(s32 v) { return (s32)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ u64 uf(u64 v) { return ((u64)uf((u32)(v >> 32)) << 32) | (u64)uf((u32)v); }
 
+// add v to statistics counter k, held in lane k of a vector register
+__device__ __forceinline__ void lcount(u32 &cvec, int k, u32 v) { cvec += (threadIdx.x == (unsigned)k) ? v : 0u; }
 __device__ __forceinline__ s32 wave_sum_i(s32 v) {
   return (s32)__builtin_amdgcn_readlane(zw::wave_scan_add(v), 63);
 }
@@ -684,7 +686,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   __shared__ u8 mflag[MSTACK];
   __shared__ u32 killers[MAXPLY + 2][2];
   __shared__ float hpos[zm::NH1];
-
   const int lane = threadIdx.x;
   const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
 
@@ -714,21 +715,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     }
 
     // ---------------------------------------------------------------- root set-up
-    u64 q[8];
-    if (serving) {
-      for (int k = 0; k < 8; k++) q[k] = uf(ag_load(&a.dreq[r].pos[k]));
-    } else {
-      for (int k = 0; k < 8; k++) q[k] = a.roots[(size_t)r * 8 + k];
-    }
-    Pos root;
-    root.b0 = q[0];
-    root.b1 = q[1];
-    root.b2 = q[2];
-    root.b3 = q[3];
-    root.cr = q[4];
-    root.stm = (u32)(q[7] & 1u);
-    const u64 roothash = q[5];
-    const u32 roothmc = (u32)(q[6] & 0xFF);
+    // (the root position is re-read at every iteration rather than held in
+    // scalar registers through the search)
+    const u64 *const rootq = serving ? (const u64 *)a.dreq[r].pos : a.roots + (size_t)r * 8;
     int hl = 0;
     if (a.hist) {
       hl = uf(a.hlen[r]);
@@ -737,8 +726,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     for (int k = lane; k < (MAXPLY + 2) * 2; k += 64) killers[k >> 1][k & 1] = 0;
     wsync();
 
+    // per-root statistics in one vector register (counter k in lane k) rather
+    // than in scalar registers, which the node state needs
+    u32 cvec = 0;
     u64 nodes = 0;
-    u32 c_probe = 0, c_hit = 0, c_cut = 0, c_store = 0, c_defer = 0, c_maxply = 0, c_mlp = 0, c_rec = 0;
     int poll = 0;
     bool aborted = false, failed = false;
     s32 final_score = 0;
@@ -746,14 +737,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
 
     for (int id = 1; id <= depth_r && !aborted && !failed; id++) {
       Node n;
-      n.p = root;
-      n.hash = roothash;
-      n.phash = hash_pieces(root);
+      {
+        u64 q[8];
+        if (serving) {
+          for (int k = 0; k < 8; k++) q[k] = uf(ag_load(&rootq[k]));
+        } else {
+          for (int k = 0; k < 8; k++) q[k] = rootq[k];
+        }
+        n.p.b0 = q[0];
+        n.p.b1 = q[1];
+        n.p.b2 = q[2];
+        n.p.b3 = q[3];
+        n.p.cr = q[4];
+        n.p.stm = (u32)(q[7] & 1u);
+        n.hash = q[5];
+        n.hmc = (u32)(q[6] & 0xFF);
+      }
+      n.phash = hash_pieces(n.p);
       n.alpha = -INF;
       n.beta = INF;
       n.depth = id;
       n.lastmove = 0;
-      n.hmc = roothmc;
       n.mbeg = 0;
       n.mcnt = 0;
       n.busy = false;
@@ -826,7 +830,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             ev0 = n.p.stm ? -e : e;
             have_ev = true;
             if (ev0 >= n.beta || sd >= MAXPLY) {
-              if (sd >= MAXPLY) c_maxply++;
+              if (sd >= MAXPLY) lcount(cvec, C_MAXPLY, 1);
               v = ev0;
               state = S_RETURN;
               wsync();
@@ -896,7 +900,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               mscore[n.mbeg + k] = (int16_t)sc;
             }
             wsync();
-            c_mlp++;
+            lcount(cvec, C_MLP_NODES, 1);
           }
           // terminal / draw tests in the reference's order (src/zeta.cl:1554-1609)
           bool draw = false;
@@ -929,7 +933,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             v = n.kic ? -INF + plyn : 0;
             state = S_RETURN;
           } else if (sd >= MAXPLY || total == 0) {
-            if (sd >= MAXPLY) c_maxply++;
+            if (sd >= MAXPLY) lcount(cvec, C_MAXPLY, 1);
             v = ev;
             state = S_RETURN;
           } else if (n.qs && !n.kic && ev >= n.beta) {
@@ -942,9 +946,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             if (a.ttmask && !n.qs && sd > 1) {
               if (!early) tte = *(const u32x4 *)(a.tt + 2 * (key & a.ttmask));
               const u64 w0 = uf(((u64)tte.y << 32) | tte.x), w1 = uf(((u64)tte.w << 32) | tte.z);
-              c_probe++;
+              lcount(cvec, C_TT_PROBE, 1);
               if ((w0 ^ w1) == key && ((w1 >> 57) & 1)) {
-                c_hit++;
+                lcount(cvec, C_TT_HIT, 1);
                 ttmove = (u32)w1;
                 s32 ts = (s32)(int16_t)(u16)(w1 >> 32);
                 const int d = (int)((w1 >> 48) & 0xFF);
@@ -952,7 +956,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
                 if (is_mate(ts) && ts < 0) ts += plyn;
                 if (((w1 >> 56) & 1) && n.hmc == 0 && d == n.depth && ts > -INF && ts < INF && ts > n.alpha) {
                   n.alpha = ts;
-                  c_cut++;
+                  lcount(cvec, C_TT_CUT, 1);
                   if (n.alpha >= n.beta) {
                     v = n.alpha;
                     state = S_RETURN;
@@ -988,7 +992,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               continue;
             }
             v = n.alpha;
-            c_store += tt_store(a, n, sd, lane);
+            lcount(cvec, C_TT_STORE, tt_store(a, n, sd, lane));
             state = S_RETURN;
             continue;
           }
@@ -1017,7 +1021,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
                 mflag[k] = 1;
               }
               n.ndefer++;
-              c_defer++;
+              lcount(cvec, C_DEFER, 1);
               wsync();
               continue;
             }
@@ -1076,14 +1080,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         }
         if (n.alpha >= n.beta) {
           if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 1 && n.mcnt >= 2 &&
-              ((u32)(n.hash >> 40) & a.rec_mask) == 0)
-            log_pairs(a, n, lane, mmove, mscore, killers[sd][0], killers[sd][1], c_rec);
+              ((u32)(n.hash >> 40) & a.rec_mask) == 0) {
+            u32 nrec = 0;
+            log_pairs(a, n, lane, mmove, mscore, killers[sd][0], killers[sd][1], nrec);
+            lcount(cvec, C_RECORDS, nrec);
+          }
           if (((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0 && killers[sd][0] != n.move) {
             killers[sd][1] = killers[sd][0];
             killers[sd][0] = n.move;
           }
           v = n.alpha;
-          c_store += tt_store(a, n, sd, lane);
+          lcount(cvec, C_TT_STORE, tt_store(a, n, sd, lane));
           state = S_RETURN;
         } else {
           state = S_NEXT;
@@ -1105,17 +1112,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     }
 
     // ---------------------------------------------------------------- publish
+    if (lane < C_NODES && cvec) atomicAdd((unsigned long long *)&a.ctr[lane], (unsigned long long)cvec);
     if (lane == 0) {
       if (!serving || sw_tk(ag_load(&a.sw[r])) == ticket)
         atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
-      atomicAdd((unsigned long long *)&a.ctr[C_TT_PROBE], (unsigned long long)c_probe);
-      atomicAdd((unsigned long long *)&a.ctr[C_TT_HIT], (unsigned long long)c_hit);
-      atomicAdd((unsigned long long *)&a.ctr[C_TT_CUT], (unsigned long long)c_cut);
-      atomicAdd((unsigned long long *)&a.ctr[C_TT_STORE], (unsigned long long)c_store);
-      atomicAdd((unsigned long long *)&a.ctr[C_DEFER], (unsigned long long)c_defer);
-      atomicAdd((unsigned long long *)&a.ctr[C_MAXPLY], (unsigned long long)c_maxply);
-      atomicAdd((unsigned long long *)&a.ctr[C_MLP_NODES], (unsigned long long)c_mlp);
-      atomicAdd((unsigned long long *)&a.ctr[C_RECORDS], (unsigned long long)c_rec);
       atomicAdd((unsigned long long *)&a.ctr[C_NODES], (unsigned long long)nodes);
       if (serving) {
         serve_leave(a, r, ticket, aborted, failed, final_score, final_move);
