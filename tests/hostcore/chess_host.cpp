@@ -47,6 +47,8 @@ extern "C" int hc_enumerate_sets(const uint64_t *b, int stm, uint32_t *moves) {
   return n;
 }
 
+extern "C" int hc_nth_bit(uint64_t x, int k) { return nth_bit(x, k); }
+
 extern "C" void hc_apply(const uint64_t *b, int stm, uint32_t m, int rook_flag, uint64_t *out) {
   Pos p = load(b, stm);
   Pos c = apply(p, m, rook_flag != 0);

@@ -25,6 +25,7 @@ def H():
     L.hc_count.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_enumerate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.hc_enumerate_sets.argtypes = [vp, ctypes.c_int, vp]
+    L.hc_nth_bit.argtypes = [ctypes.c_uint64, ctypes.c_int]
     L.hc_perft.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_perft.restype = ctypes.c_uint64
     L.hc_enumerate_lane.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -115,6 +116,17 @@ def test_grouped_enumeration_matches_oracle(H):
             _grouped_vs_oracle(H, c, buf)
             for m2 in _oracle_moves(c, 0)[:40]:
                 _grouped_vs_oracle(H, R.domove(c, m2), buf)
+
+
+def test_nth_bit(H):
+    # the search's move-list builder takes the k-th target of a lane's set
+    rng = np.random.default_rng(11)
+    xs = [1, 1 << 63, (1 << 64) - 1, 0x8000000100000001] + [
+        int(rng.integers(1, 2**63, dtype=np.int64)) ^ (int(rng.integers(0, 2, dtype=np.int64)) << 63) for _ in range(2000)]
+    for x in xs:
+        bits = [i for i in range(64) if (x >> i) & 1]
+        for k, b in enumerate(bits):
+            assert H.hc_nth_bit(x, k) == b, (hex(x), k)
 
 
 def test_single_square_sliders_equal_fills(H):

@@ -451,6 +451,29 @@ ZD u64 piece_targets(const Pos &p, const Legal &L, int sq, bool qs) {
   }
 }
 
+// Index of the k-th set bit (k from 0) of x, which has more than k set bits:
+// a binary search on popcounts, with no loop over the bits.
+ZD int nth_bit(u64 x, int k) {
+  u32 w = (u32)x;
+  int base = 0;
+  const int c32 = popc((u64)w);
+  if (k >= c32) {
+    k -= c32;
+    w = (u32)(x >> 32);
+    base = 32;
+  }
+#pragma unroll
+  for (int s = 16; s >= 1; s >>= 1) {
+    const int c = popc((u64)(w & ((1u << s) - 1u)));
+    if (k >= c) {
+      k -= c;
+      w >>= s;
+      base += s;
+    }
+  }
+  return base;
+}
+
 // ---- node analysis with one lane per square (the search's wave-per-node
 // layout).  The uniform part is cheap (king-square attackers); everything
 // set-wise is replaced by per-square questions answered by all 64 lanes at

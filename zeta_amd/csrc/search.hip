@@ -686,6 +686,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   __shared__ u8 mflag[MSTACK];
   __shared__ u32 killers[MAXPLY + 2][2];
   __shared__ float hpos[zm::NH1];
+  __shared__ u8 mstart[64];  // move-list start marks (lane + 1 at its offset)
   const int lane = threadIdx.x;
   const int hstart = (int)((blockIdx.x * 7919u) % (unsigned)(a.nroots > 0 ? a.nroots : 1));
 
@@ -856,12 +857,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             break;
           }
           {
-            int k = n.mbeg + off;
             const u32 k0 = killers[sd][0], k1 = killers[sd][1];
-            while (t) {
-              const int to = lsb(t);
-              t &= t - 1;
-              const u32 m = make_code(n.p, lane, to);
+            auto put = [&](int k, int from, int to) {
+              const u32 m = make_code(n.p, from, to);
               s32 sc = order_score(n.p.stm, m);
               sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
               if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
@@ -871,7 +869,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               mmove[k] = m;
               mscore[k] = (int16_t)sc;
               mflag[k] = 0;
-              k++;
+            };
+            if (!n.qs && total <= 64) {
+              // lane j writes move j of the list (the same order as the loop
+              // below: origin lanes ascending, then targets ascending): its
+              // origin is the last lane with targets whose offset is <= j (a
+              // prefix max over start marks), its target the (j - offset)-th
+              // bit of that lane's set -- one pass instead of as many loop
+              // trips as the busiest lane has targets
+              mstart[lane] = 0;
+              wsync();
+              if (t) mstart[off] = (u8)(lane + 1);
+              wsync();
+              const int own1 = (int)zw::wave_scan_max(lane < total ? (u32)mstart[lane] : 0u);
+              const int from = (own1 > 0 ? own1 : 1) - 1;
+              const int k = lane - __shfl(off, from, 64);
+              const u64 ts = ((u64)(u32)__shfl((int)(u32)(t >> 32), from, 64) << 32) | (u32)__shfl((int)(u32)t, from, 64);
+              if (lane < total) put(n.mbeg + lane, from, nth_bit(ts, k));
+            } else {
+              int k = n.mbeg + off;
+              while (t) {
+                const int to = lsb(t);
+                t &= t - 1;
+                put(k, lane, to);
+                k++;
+              }
             }
           }
           if (mlp_r && !n.qs && n.depth >= a.mlp_min_depth && total > 1) {
