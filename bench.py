@@ -39,7 +39,7 @@ VALU_LANE_INSTS_PER_POSITION = 1022.6
 HBM_BYTES_PER_POSITION = 12.85
 # k_search: one node per wavefront; SQ_INSTS_VALU / nodes over the depth-5
 # search leg (profiles/search_c3_pmc.md)
-VALU_WAVE_INSTS_PER_NODE = 980.0
+VALU_WAVE_INSTS_PER_NODE = 915.8
 
 
 def cpu_baseline(pos, depth, threads):
