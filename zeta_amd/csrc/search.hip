@@ -736,7 +736,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     s32 final_score = 0;
     u32 final_move = 0;
 
-    for (int id = 1; id <= depth_r && !aborted && !failed; id++) {
+    // a worker joining a root that others are already searching starts at the
+    // iteration in progress: the earlier ones would only repeat work whose
+    // moves are in the shared TT (the final iteration alone sets the score)
+    int id0 = 1;
+    if (lane == 0) id0 = __atomic_load_n(&a.depth_done[r], __ATOMIC_RELAXED) + 1;
+    id0 = uf(id0);
+    id0 = id0 < 1 ? 1 : (id0 > depth_r ? depth_r : id0);
+    for (int id = id0; id <= depth_r && !aborted && !failed; id++) {
       Node n;
       {
         u64 q[8];
