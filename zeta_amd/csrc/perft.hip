@@ -149,9 +149,13 @@ __global__ __launch_bounds__(TPB) void k_scan_down(const u32 *__restrict__ cnt, 
 constexpr int XTPB = 128;
 constexpr int XCAP = 4096;  // 16 KiB of codes + 4 KiB of lane indices per block
 __global__ __launch_bounds__(XTPB) void k_expand(Front f, size_t i0, size_t i1, const u64 *__restrict__ off,
-                                                 u64 base, u64 *__restrict__ pm) {
+                                                 u64 base, u64 *__restrict__ pm, const u64 *__restrict__ gtotal,
+                                                 u64 gcap) {
   __shared__ u32 lcode[XCAP];
   __shared__ u8 llane[XCAP];
+  // launched before the host knows the level's pair total: the whole grid
+  // leaves when the total does not fit the pair buffer (the host relaunches)
+  if (gtotal && *gtotal > gcap) return;
   const size_t bi0 = i0 + (size_t)blockIdx.x * XTPB;
   const size_t bi1 = bi0 + XTPB < i1 ? bi0 + XTPB : i1;
   const u64 blo = off[bi0], total = off[bi1] - blo;
@@ -346,6 +350,8 @@ struct LevelBufs {
 struct PerftState {
   LevelBufs lev[MAXLEV];
   zr::DevBuf total, roots, counts, fails, cuts;
+  u64 *h_total = nullptr;          // pinned host copy of a level's pair total
+  hipEvent_t ev_total = nullptr;   // recorded after that copy
 };
 PerftState g_ps;
 
@@ -394,8 +400,20 @@ int scan_counts(int L, size_t i0, size_t i1, u64 *total) {
   ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_reduce, (unsigned)nb, TPB, 0, cnt, n, (u64 *)b.bsum.p);
   ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_top, 1, TPB, 0, (u64 *)b.bsum.p, nb, (u64 *)g_ps.total.p);
   ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_down, (unsigned)nb, TPB, 0, cnt, n, (const u64 *)b.bsum.p, (u64 *)b.off.p);
-  ZCHECK(hipMemcpyAsync(total, g_ps.total.p, 8, hipMemcpyDeviceToHost, zr::ctx().stream));
-  ZCHECK(hipStreamSynchronize(zr::ctx().stream));
+  if (!g_ps.h_total) {
+    ZCHECK(hipHostMalloc((void **)&g_ps.h_total, 8, hipHostMallocDefault));
+    ZCHECK(hipEventCreateWithFlags(&g_ps.ev_total, hipEventDisableTiming));
+  }
+  ZCHECK(hipMemcpyAsync(g_ps.h_total, g_ps.total.p, 8, hipMemcpyDeviceToHost, zr::ctx().stream));
+  ZCHECK(hipEventRecord(g_ps.ev_total, zr::ctx().stream));
+  *total = ~0ULL;  // known after wait_total()
+  return ZETA_OK;
+}
+
+// the total copied by scan_counts (the stream may run on: only the copy is waited for)
+int wait_total(u64 *total) {
+  ZCHECK(hipEventSynchronize(g_ps.ev_total));
+  *total = *g_ps.h_total;
   return ZETA_OK;
 }
 
@@ -403,8 +421,18 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
   u64 total = 0;
   int r = scan_counts(L, i0, i1, &total);
   if (r) return r;
-  if (total == 0) return ZETA_OK;
+  if (i1 == i0) return ZETA_OK;
   const u64 budget = zr::ctx().cfg.frontier_budget;
+  // the expansion is queued before the total is known and writes into the
+  // current pair buffer when the total fits it (k_expand checks), so the
+  // host's wait for the total overlaps the expansion instead of preceding it
+  const u64 spec_cap = g_ps.lev[L].paircap < budget ? g_ps.lev[L].paircap : budget;
+  if (spec_cap)
+    ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, XTPB), XTPB, 0, front_of(L), i0, i1,
+            (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p, (const u64 *)g_ps.total.p,
+            spec_cap);
+  if ((r = wait_total(&total))) return r;
+  if (total == 0) return ZETA_OK;
   // a level with more pairs than the budget runs in chunks of parents, each
   // with at most `budget` pairs; the cuts come from the device (k_cuts) so the
   // host reads a few words, not the level's offsets
@@ -428,7 +456,7 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
       if (m == 0) continue;
       if ((r = ensure_pairs(L, m))) return r;
       ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, XTPB), XTPB, 0, front_of(L), i0 + a0, i0 + a1,
-              (const u64 *)g_ps.lev[L].off.p - i0, coff[c], (u64 *)g_ps.lev[L].pm.p);
+              (const u64 *)g_ps.lev[L].off.p - i0, coff[c], (u64 *)g_ps.lev[L].pm.p, (const u64 *)nullptr, (u64)0);
       if (L + 1 == D - 1) {
         ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u64 *)g_ps.lev[L].pm.p,
                 (size_t)m, d_counts, nobulk);
@@ -441,9 +469,11 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
     }
     return ZETA_OK;
   }
-  if ((r = ensure_pairs(L, total))) return r;
-  ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, XTPB), XTPB, 0, front_of(L), i0, i1,
-          (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p);
+  if (!spec_cap || total > spec_cap) {
+    if ((r = ensure_pairs(L, total))) return r;
+    ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, XTPB), XTPB, 0, front_of(L), i0, i1,
+            (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p, (const u64 *)nullptr, (u64)0);
+  }
   if (L + 1 == D - 1) {
     ZLAUNCH(zr::K_PERFT_LEAF, total, k_leaf, blocks(total, TPB), TPB, 0, front_of(L),
             (const u64 *)g_ps.lev[L].pm.p, (size_t)total, d_counts, nobulk);
@@ -492,6 +522,10 @@ void perft_release() {
   zr::release(g_ps.counts);
   zr::release(g_ps.fails);
   zr::release(g_ps.cuts);
+  if (g_ps.h_total) (void)hipHostFree(g_ps.h_total);
+  g_ps.h_total = nullptr;
+  if (g_ps.ev_total) (void)hipEventDestroy(g_ps.ev_total);
+  g_ps.ev_total = nullptr;
 }
 
 extern "C" int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts, int flags) {
