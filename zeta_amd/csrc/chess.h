@@ -9,8 +9,9 @@
 //   bit of ranks 4/5).
 //
 // The generator here is set-wise, not the reference's 64-lane square-wise one:
-// one lane analyses a whole position with Kogge-Stone occluded fills (opponent
-// attack map with the own king transparent, king rays for checkers and pins),
+// one lane analyses a whole position with Kogge-Stone occluded fills (the
+// opponent's slider rays give the attack map, and against the king's own rays
+// the checkers, check rays, pins and the squares attacked through the king),
 // then counts legal moves direction by direction (every target of a
 // direction/offset has exactly one origin, so popcounts count moves).  It
 // generates exactly the reference's legal move set, including its quirks:
@@ -240,7 +241,7 @@ ZD u64 bishop_sq(int sq, u64 occ) {
 struct Legal {
   u64 own, opp, occ, kbb;
   u64 pawns, knights, kings, rq, bq;
-  u64 A;         // opponent attacks, own king transparent
+  u64 A;         // opponent attacks as if the own king were not on the board
   u64 checkers;  // opponent pieces giving check
   u64 cmask;     // allowed targets of non-king moves: ~0, block/capture set, or 0 (double check)
   u64 pinR, pinF, pinD, pinA;  // own pieces pinned on the king's rank, file, diagonal, anti-diagonal
