@@ -27,6 +27,23 @@ def test_terminal_and_draw_positions(zeta, oracle):
             assert (got["status"] == 0).all()
 
 
+def test_many_move_positions(zeta, oracle):
+    # nodes with more than 64 moves build their move list in the per-lane
+    # loop, smaller interior nodes one move per lane: both against the oracle
+    fens = ["R6R/3Q4/1Q4Q1/4Q3/2Q4Q/Q4Q2/pp1Q4/kBNN1KB1 w - - 0 1",  # 218 moves
+            "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1",
+            "3k4/8/8/8/8/8/8/QQQQKQQQ w - - 0 1",  # 101 moves
+            "3K4/8/8/8/8/8/8/qqqqkqqq b - - 0 1"]
+    pos = np.concatenate([oracle.setboard(f) for f in fens])
+    for d in (1, 2, 3):
+        want = [oracle.search(pos[i:i + 1], d)[0] for i in range(len(pos))]
+        for workers, tt in ((1, 0), (4, 20)):
+            got = zeta.search(pos, d, workers=workers, tt_log2=tt)
+            print("depth", d, "workers", workers, "tt", tt, list(got["score"]), want, flush=True)
+            assert list(got["score"]) == want
+            assert (got["status"] == 0).all()
+
+
 @pytest.mark.parametrize("depth", [1, 2, 3, 4])
 def test_playout_positions_small_depth(zeta, oracle, depth):
     pos = oracle.gen_playouts(3, 128, 20, 60)
