@@ -9,8 +9,8 @@
  * the reference file:line it follows.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, and only as the checker or the timed CPU baseline.  The
- * product (libzetahip.so) never links, loads or calls it.
+ * load this library, and only to check device results or as the timed CPU
+ * baseline.  The product (libzetahip.so) never links, loads or calls it.
  */
 #ifndef ZETA_REFCPU_H
 #define ZETA_REFCPU_H

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/librefcpu.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker and by bench.py's cpu_baseline leg.  The product never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check the device results, and by bench.py's cpu_baseline leg.  The product
+never imports it.
 """
 import ctypes
 import os
