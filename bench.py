@@ -120,7 +120,11 @@ def main():
                 "kernel": "perft_leaf", "avg_launch_ms": round(leaf[1] / leaf[0], 4),
                 "items_per_launch": int(per_launch_items),
                 "valu_lane_insts_per_item": VALU_LANE_INSTS_PER_POSITION,
-                "hbm_GBps": round(per_launch_items * HBM_BYTES_PER_POSITION / avg_s / 1e9, 1)}
+                "hbm_GBps": round(per_launch_items * HBM_BYTES_PER_POSITION / avg_s / 1e9, 1),
+                # the HBM roofline of the same kernel, for comparison: far from binding
+                "hbm": {"achieved": round(per_launch_items * HBM_BYTES_PER_POSITION / avg_s / 1e9, 1),
+                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": round(per_launch_items * HBM_BYTES_PER_POSITION / avg_s / HBM_PEAK, 4)}}
 
     calib = None
     if args.calib_copy:
