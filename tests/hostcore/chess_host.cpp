@@ -49,6 +49,36 @@ extern "C" int hc_enumerate_sets(const uint64_t *b, int stm, uint32_t *moves) {
 
 extern "C" int hc_nth_bit(uint64_t x, int k) { return nth_bit(x, k); }
 
+// castle_targets against the per-square form of the same rule (king-relative
+// squares k-4..k+3, each its own shift), for arbitrary king squares, moved
+// flags, occupancy, attack map and rooks: the fast path (shifted constants for
+// kings on squares 4-60) must give the same targets
+extern "C" uint64_t hc_castle(uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3, uint64_t cr, int stm, int ksq,
+                              uint64_t occ, uint64_t own, uint64_t A) {
+  Pos p;
+  p.b0 = b0; p.b1 = b1; p.b2 = b2; p.b3 = b3; p.cr = cr; p.stm = (u32)stm;
+  Legal L{};
+  L.ksq = ksq; L.occ = occ; L.own = own; L.A = A;
+  L.rq = b3 & (b1 ^ b2);
+  return castle_targets(p, L);
+}
+
+extern "C" uint64_t hc_castle_per_square(uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3, uint64_t cr, int stm,
+                                         int k, uint64_t occ, uint64_t own, uint64_t A) {
+  (void)b0;
+  if (!(cr & SMCRALL)) return 0;
+  const bool okq = stm ? ((~cr & SMCRBLACKQ) == SMCRBLACKQ) : ((~cr & SMCRWHITEQ) == SMCRWHITEQ);
+  const bool okk = stm ? ((~cr & SMCRBLACKK) == SMCRBLACKK) : ((~cr & SMCRWHITEK) == SMCRWHITEK);
+  const u64 ownrooks = own & (b3 & (b1 ^ b2)) & ~b2;
+  u64 t = 0;
+  if (okq && (ownrooks & bit(k - 4)) && !(occ & (bit(k - 1) | bit(k - 2) | bit(k - 3))) &&
+      !(A & (bit(k) | bit(k - 1) | bit(k - 2))))
+    t |= bit(k - 2);
+  if (okk && (ownrooks & bit(k + 3)) && !(occ & (bit(k + 1) | bit(k + 2))) && !(A & (bit(k) | bit(k + 1) | bit(k + 2))))
+    t |= bit(k + 2);
+  return t;
+}
+
 extern "C" void hc_apply(const uint64_t *b, int stm, uint32_t m, int rook_flag, uint64_t *out) {
   Pos p = load(b, stm);
   Pos c = apply(p, m, rook_flag != 0);

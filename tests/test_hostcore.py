@@ -26,6 +26,11 @@ def H():
     L.hc_enumerate.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.hc_enumerate_sets.argtypes = [vp, ctypes.c_int, vp]
     L.hc_nth_bit.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    u64 = ctypes.c_uint64
+    L.hc_castle.argtypes = [u64, u64, u64, u64, u64, ctypes.c_int, ctypes.c_int, u64, u64, u64]
+    L.hc_castle.restype = u64
+    L.hc_castle_per_square.argtypes = L.hc_castle.argtypes
+    L.hc_castle_per_square.restype = u64
     L.hc_perft.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.hc_perft.restype = ctypes.c_uint64
     L.hc_enumerate_lane.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -127,6 +132,32 @@ def test_nth_bit(H):
         bits = [i for i in range(64) if (x >> i) & 1]
         for k, b in enumerate(bits):
             assert H.hc_nth_bit(x, k) == b, (hex(x), k)
+
+
+def test_castle_shifted_constants(H):
+    # castle_targets' fast path (kings on squares 4-60) against the per-square
+    # rule, on random boards, moved flags and attack maps with the castling
+    # squares often empty, the rooks often home and the flags often clear
+    rng = np.random.default_rng(13)
+    r64 = lambda: int(rng.integers(0, 2**63, dtype=np.int64)) | (int(rng.integers(0, 2, dtype=np.int64)) << 63)
+    hits = 0
+    for _ in range(20000):
+        stm = int(rng.integers(0, 2))
+        k = int(rng.choice([4, 60, int(rng.integers(0, 64))]))
+        rank = 0xFF << (56 if stm else 0)
+        occ = r64() & r64() & ~(rank if rng.integers(0, 2) else 0)
+        A = r64() & r64() & r64() & ~(rank if rng.integers(0, 2) else 0)
+        rooks = (0x81 << (56 if stm else 0)) if rng.integers(0, 2) else r64()
+        own = rooks | (r64() & occ)
+        b3, b1, b2 = rooks | (r64() & occ), rooks | (r64() & occ), r64() & occ & ~rooks
+        cr = r64() & r64() & r64()
+        if rng.integers(0, 2):
+            cr |= 1 << 59  # a flagged square, so the precondition holds
+        args = (0, b1, b2, b3, cr, stm, k, occ | rooks, own, A)
+        want = H.hc_castle_per_square(*args)
+        hits += want != 0
+        assert H.hc_castle(*args) == want, args
+    assert hits > 100
 
 
 def test_single_square_sliders_equal_fills(H):
