@@ -1040,11 +1040,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             u32 b = 0;
             if (lane == 0) b = __atomic_load_n(&a.busy[bidx], __ATOMIC_RELAXED);
             b = uf(b);
-            // the first child too: a worker that finds it busy moves on to the
-            // siblings and comes back to it in the second pass, which keeps
-            // helpers off the path every worker starts on (10 % shorter time
-            // to depth on the 1024-root depth-6 batch)
-            if (n.iter == 1 && b) {
+            // batch searches defer the first child too: a worker that finds
+            // it busy moves on to the siblings and comes back to it in the
+            // second pass, which keeps helpers off the path every worker
+            // starts on (10-15 % shorter time to depth on the 1024-root
+            // depth-6 batch).  Served requests, deep and with many workers
+            // on one root, defer only later siblings: there the first-child
+            // deferral sends helpers into siblings searched with poor bounds
+            // (a depth 4-9 stream completed 43 instead of 47 of 48 requests
+            // within 120 s with it)
+            if (n.iter == 1 && b && (!serving || n.searched > 0)) {
               if (lane == 0) {
                 mscore[k] = (int16_t)sk;
                 mflag[k] = 1;
