@@ -1,0 +1,39 @@
+// Host build of the CECP front end's board I/O (zeta_amd/host/zeta_engine.cpp)
+// for CPU tests: FEN in (setboard), moves in coordinate notation (can2move,
+// domove), FEN out (createfen).  The front end's main is renamed and never
+// called; nothing here touches a device.
+#define main zeta_engine_main
+#include "../../zeta_amd/host/zeta_engine.cpp"
+#undef main
+
+#include <cstring>
+
+// Sets up `fen`, plays the space-separated coordinate moves in `moves`, and
+// writes the resulting FEN to out[cap].  Returns 0, or -1 for a FEN the
+// parser rejects, -2 for a move that is not legal in its position, -3 for an
+// output buffer that is too small.
+extern "C" int fh_play(const char *fen, const char *moves, char *out, int cap) {
+  uint64_t b[7];
+  bool stm;
+  int64_t gp = 0;
+  if (!setboard(b, stm, fen, &gp)) return -1;
+  int32_t ply = 0;
+  const char *s = moves;
+  while (s && *s) {
+    while (*s == ' ') ++s;
+    if (!*s) break;
+    char mv[8] = {0};
+    int k = 0;
+    while (s[k] && s[k] != ' ' && k < 7) { mv[k] = s[k]; ++k; }
+    s += k;
+    const uint32_t m = can2move(mv, b, stm);
+    if (!m) return -2;
+    domove(b, stm, m);
+    stm = !stm;
+    ++ply;
+  }
+  const std::string f = createfen(b, stm, gp, ply);
+  if ((int)f.size() + 1 > cap) return -3;
+  std::memcpy(out, f.c_str(), f.size() + 1);
+  return 0;
+}

@@ -1,0 +1,91 @@
+"""CPU tests of the front end's board I/O (zeta_amd/host/zeta_engine.cpp,
+setboard / createfen after src/io.c:227-532): FEN round trips, and the FEN
+after coordinate moves played on the board, against the standard FEN of the
+same game (castle rights, e.p. target, half-move clock, move number)."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SO = os.path.join(HERE, "hostcore", "libfenhost.so")
+SRC = os.path.join(HERE, "hostcore", "fen_host.cpp")
+DEPS = [SRC, os.path.join(ROOT, "zeta_amd", "host", "zeta_engine.cpp"),
+        os.path.join(ROOT, "zeta_amd", "host", "clrun_zeta.cpp"),
+        os.path.join(ROOT, "zeta_amd", "csrc", "chess.h")]
+LIBDIR = os.path.join(ROOT, "zeta_amd")
+
+
+@pytest.fixture(scope="module")
+def F():
+    if not os.path.exists(os.path.join(LIBDIR, "libzetahip.so")):
+        pytest.skip("libzetahip.so not built")
+    if not os.path.exists(SO) or os.path.getmtime(SO) < max(os.path.getmtime(p) for p in DEPS):
+        subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas",
+                               "-I" + os.path.join(ROOT, "include"), "-o", SO, SRC,
+                               os.path.join(ROOT, "zeta_amd", "host", "clrun_zeta.cpp"),
+                               "-L" + LIBDIR, "-lzetahip", "-Wl,-rpath," + LIBDIR])
+    L = ctypes.CDLL(SO)
+    L.fh_play.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    L.fh_play.restype = ctypes.c_int
+
+    def play(fen, moves=""):
+        out = ctypes.create_string_buffer(256)
+        rc = L.fh_play(fen.encode(), moves.encode(), out, 256)
+        assert rc == 0, (fen, moves, rc)
+        return out.value.decode()
+    return play
+
+
+START = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
+
+
+@pytest.mark.parametrize("fen", [
+    START,
+    "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1",
+    "8/2p5/3p4/KP5r/1R3p1k/8/4P1P1/8 w - - 0 1",
+    "r3k2r/Pppp1ppp/1b3nbN/nP6/BBP1P3/q4N2/Pp1P2PP/R2Q1RK1 w kq - 0 1",
+    "rnbq1k1r/pp1Pbppp/2p5/8/2B5/8/PPP1NnPP/RNBQK2R w KQ - 1 8",
+    "r4rk1/1pp1qppp/p1np1n2/2b1p1B1/2B1P1b1/P1NP1N2/1PP1QPPP/R4RK1 w - - 0 10",
+    "r3k2r/8/8/8/8/8/8/R3K2R b Qk - 5 40",
+])
+def test_roundtrip(F, fen):
+    assert F(fen) == fen
+
+
+@pytest.mark.parametrize("fen,moves,want", [
+    (START, "e2e4", "rnbqkbnr/pppppppp/8/8/4P3/8/PPPP1PPP/RNBQKBNR b KQkq e3 0 1"),
+    (START, "e2e4 d7d5", "rnbqkbnr/ppp1pppp/8/3p4/4P3/8/PPPP1PPP/RNBQKBNR w KQkq d6 0 2"),
+    (START, "g1f3", "rnbqkbnr/pppppppp/8/8/8/5N2/PPPPPPPP/RNBQKB1R b KQkq - 1 1"),
+    (START, "e2e4 e7e5 e1e2", "rnbqkbnr/pppp1ppp/8/4p3/4P3/8/PPPPKPPP/RNBQ1BNR b kq - 1 2"),
+    (START, "g1f3 g8f6 h1g1", "rnbqkb1r/pppppppp/5n2/8/8/5N2/PPPPPPPP/RNBQKBR1 b Qkq - 3 2"),
+    (START, "e2e4 d7d5 e4d5", "rnbqkbnr/ppp1pppp/8/3P4/8/8/PPPP1PPP/RNBQKBNR b KQkq - 0 2"),
+    # e.p. capture: the captured pawn leaves d5
+    (START, "e2e4 a7a6 e4e5 d7d5 e5d6", "rnbqkbnr/1pp1pppp/p2P4/8/8/8/PPPP1PPP/RNBQKBNR b KQkq - 0 3"),
+    # castling moves king and rook and ends that side's rights; unlike standard
+    # FEN the reference resets the half-move clock on a castle
+    # (src/bitboard.c:664,681; src/zeta.cl:2368-2373)
+    ("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 3 1", "e1g1",
+     "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R4RK1 b kq - 0 1"),
+    ("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R b KQkq - 3 1", "e8c8",
+     "2kr3r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQ - 0 2"),
+    # a rook captured on its home square takes that side's right with it
+    ("r3k2r/8/8/8/8/8/8/R3K2R w KQkq - 0 1", "a1a8", "R3k2r/8/8/8/8/8/8/4K2R b Kk - 0 1"),
+    # promotion is always to a queen
+    ("8/P7/8/8/8/8/8/k6K w - - 0 1", "a7a8", "Q7/8/8/8/8/8/8/k6K b - - 0 1"),
+])
+def test_fen_after_moves(F, fen, moves, want):
+    assert F(fen, moves) == want
+
+
+def test_fen_ep_square_not_kept_from_input(F):
+    # the reference's setboard marks a FEN's e.p. square on rank 3/6, where
+    # neither its move generator nor its createfen looks (src/io.c:487-496,
+    # 292-310): the square reads back as "-" and the e.p. capture is not legal
+    fen = "rnbqkbnr/ppp1pppp/8/3pP3/8/8/PPPP1PPP/RNBQKBNR w KQkq d6 0 3"
+    assert F(fen) == fen.replace(" d6 ", " - ")
+    out = ctypes.create_string_buffer(256)
+    L = ctypes.CDLL(SO)
+    assert L.fh_play(fen.encode(), b"e5d6", out, 256) == -2

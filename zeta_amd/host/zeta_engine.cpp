@@ -37,6 +37,7 @@ uint64_t tt2_memory = 0;
 int32_t opencl_device_id = 0;
 int32_t opencl_platform_id = 0;
 int32_t PLY = 0;
+int64_t GAMEPLY = 0;  // src/zeta.c:89: the game ply at the last setboard (2 * fullmove + stm)
 uint64_t *GLOBAL_BOARD = nullptr;
 uint64_t *GLOBAL_HASHHISTORY = nullptr;
 uint64_t *COUNTERS = nullptr;
@@ -170,7 +171,9 @@ void domove(uint64_t *b, bool stm, uint32_t m) {
 // FEN -> quad-bitboard with the reference's rules (src/io.c:335-532): moved
 // flags start all set, castle letters clear rook and king squares, the FEN
 // e.p. square is cleared on rank 3/6 where the move generator never looks.
-bool setboard(uint64_t *b, bool &stm, const char *fen) {
+// gameply (optional): the FEN's move number as the reference counts it,
+// 2 * fullmove + stm (src/io.c:500)
+bool setboard(uint64_t *b, bool &stm, const char *fen, int64_t *gameply = nullptr) {
   char position[1024] = {0}, cstm[16] = {0}, castle[16] = {0}, cep[16] = {0};
   unsigned long long hmc = 0, fullmove = 1;
   if (sscanf(fen, "%1023s %15s %15s %15s %llu %llu", position, cstm, castle, cep, &hmc, &fullmove) < 1)
@@ -226,6 +229,7 @@ bool setboard(uint64_t *b, bool &stm, const char *fen) {
   nb[5] = zc::hash_full(to_pos(nb, nstm));
   memcpy(b, nb, sizeof nb);
   stm = nstm;
+  if (gameply) *gameply = (int64_t)fullmove * 2 + (nstm ? 1 : 0);
   return true;
 }
 
@@ -251,19 +255,66 @@ uint32_t can2move(const char *s, const uint64_t *b, bool stm) {
   return 0;
 }
 
-void printboard(const uint64_t *b) {
-  static const char *chars = ".PNKBRQ.pnkbrq";
+// the board as a FEN string (src/io.c:227-330): castle rights from the moved
+// flags ("-" once every flagged square has moved), the e.p. target from the
+// cleared flag on ranks 4/5, the move number as (gameply + ply) / 2
+std::string createfen(const uint64_t *b, bool stm, int64_t gameply, int32_t ply) {
+  static const char *wp = " PNKBRQ", *bp = " pnkbrq";
+  std::string s;
+  for (int r = 7; r >= 0; r--) {
+    int spaces = 0;
+    for (int f = 0; f < 8; f++) {
+      const uint32_t pc = zc::piece_at(to_pos(b, false), r * 8 + f);
+      const uint32_t type = (pc >> 1) & 7;
+      if (type && spaces) {
+        s += std::to_string(spaces);
+        spaces = 0;
+      }
+      if (type) s += (pc & 1) ? bp[type] : wp[type];
+      else spaces++;
+    }
+    if (spaces) s += std::to_string(spaces);
+    if (r > 0) s += "/";
+  }
+  s += stm ? " b " : " w ";
+  const uint64_t ncr = ~b[4];
+  if ((ncr & zc::SMCRALL) == 0) {
+    s += "-";
+  } else {
+    if ((ncr & zc::SMCRWHITEK) == zc::SMCRWHITEK) s += "K";
+    if ((ncr & zc::SMCRWHITEQ) == zc::SMCRWHITEQ) s += "Q";
+    if ((ncr & zc::SMCRBLACKK) == zc::SMCRBLACKK) s += "k";
+    if ((ncr & zc::SMCRBLACKQ) == zc::SMCRBLACKQ) s += "q";
+  }
+  s += " ";
+  const uint64_t ep = ncr & 0x000000FFFF000000ULL;
+  int sq = ep ? __builtin_ctzll(ep) : 0;
+  if (sq) {
+    sq += stm ? -8 : 8;
+    s += (char)('a' + (sq & 7));
+    s += (char)('1' + (sq >> 3));
+  } else {
+    s += "-";
+  }
+  s += " " + std::to_string((unsigned long long)b[6]) + " " + std::to_string((long long)((gameply + ply) / 2));
+  return s;
+}
+
+// src/io.c:164-226: the board, "-" for an empty square, then its FEN
+void printboard(const uint64_t *b, bool stm, int64_t gameply, int32_t ply) {
+  static const char *wp = "-PNKBRQ", *bp = "-pnkbrq";
   std::string s = "###ABCDEFGH###\n";
   for (int r = 7; r >= 0; r--) {
     s += "#" + std::to_string(r + 1) + " ";
     for (int f = 0; f < 8; f++) {
-      const int sq = r * 8 + f;
-      const uint32_t pc = zc::piece_at(to_pos(b, false), sq);
-      const uint32_t type = (pc >> 1) & 7, col = pc & 1;
-      s += type ? chars[type + (col ? 7 : 0)] : '.';
+      const uint32_t pc = zc::piece_at(to_pos(b, false), r * 8 + f);
+      const uint32_t type = (pc >> 1) & 7;
+      s += (pc & 1) ? bp[type] : wp[type];
     }
     s += "\n";
   }
+  s += "###ABCDEFGH###\n";
+  s += "#fen: " + createfen(b, stm, gameply, ply) + "\n";
   out("%s", s.c_str());
 }
 
@@ -364,13 +415,15 @@ void pop_move() {
 void new_game(const char *fen) {
   bool stm = false;
   uint64_t b[7];
-  if (!setboard(b, stm, fen)) {
+  int64_t gp = 0;
+  if (!setboard(b, stm, fen, &gp)) {
     out("Error (in setting chess psotition via fen string): setboard\n");
     out("tellusererror (Error in setting chess psotition via fen string): setboard\n");
     return;
   }
   memcpy(BOARD, b, sizeof BOARD);
   STM = stm;
+  GAMEPLY = gp;
   PLY = 0;
   history.clear();
   HashHistory.assign(MAXGAMEPLY + 1, 0);
@@ -407,7 +460,7 @@ void think() {
   } else {
     out("move %s\n", move2can(move).c_str());
     push_move(move);
-    if (!xboard_mode || xboard_debug) printboard(BOARD);
+    if (!xboard_mode || xboard_debug) printboard(BOARD, STM, GAMEPLY, PLY);
     time_control(now_ms() - start);
   }
 }
@@ -462,13 +515,14 @@ bool selftest() {
   uint64_t passed = 0;
   uint64_t b[7];
   bool stm = false;
+  int64_t gp = 0;
   for (uint64_t done = 0; done < todo; done++) {
     out("#> doing perft depth: %d for position %" PRIu64 " of %" PRIu64 "\n", KATS[done].depth, done + 1, todo);
-    if (!setboard(b, stm, KATS[done].fen)) {
+    if (!setboard(b, stm, KATS[done].fen, &gp)) {
       out("# Error (in setting fen position): setboard\n");
       continue;
     }
-    printboard(b);
+    printboard(b, stm, gp, 0);
     const double start = now_ms();
     perft(b, stm, KATS[done].depth);
     const double el = (now_ms() - start) / 1000;
@@ -612,13 +666,13 @@ void xboard() {
       new_game(STARTFEN);
       SD = MAXPLY;
       MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
-      if (!xboard_mode) printboard(BOARD);
+      if (!xboard_mode) printboard(BOARD, STM, GAMEPLY, PLY);
       xboard_force = false;
     } else if (!strcmp(cmd, "setboard")) {
       const char *fen = line + strlen("setboard");
       while (*fen == ' ') fen++;
       new_game(fen);
-      if (!xboard_mode) printboard(BOARD);
+      if (!xboard_mode) printboard(BOARD, STM, GAMEPLY, PLY);
     } else if (!strcmp(cmd, "go")) {
       xboard_force = false;
       think();
@@ -663,7 +717,7 @@ void xboard() {
         continue;
       }
       push_move(m);
-      if (!xboard_mode || xboard_debug) printboard(BOARD);
+      if (!xboard_mode || xboard_debug) printboard(BOARD, STM, GAMEPLY, PLY);
       if (!xboard_force) think();
     } else if (!strcmp(cmd, "undo")) {
       if (PLY > 0) pop_move();
