@@ -168,9 +168,22 @@ def main():
                   **ctr}
 
     cpu = None
+    c1_cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         cpu = cpu_baseline(pos, args.depth, threads)
+        # config 1 on one core: the reference runs it as one work-group; the
+        # oracle's move generator is that kernel's 64-lane square-wise
+        # generator, one lane after the other
+        from oracle import refcpu
+        best = None
+        for _ in range(3):
+            t4 = time.perf_counter()
+            cc = int(refcpu.perft_batch(start, 5, 1)[0])
+            dt4 = time.perf_counter() - t4
+            best = dt4 if best is None else min(best, dt4)
+        c1_cpu = {"value": cc / best, "unit": "leaves/s", "cores": 1, "kind": "port", "leaves": cc,
+                  "sample": "oracle startpos perft(5) on one thread, best of 3 (%.3f s)" % best}
         if search is not None:
             # the oracle's alpha-beta (same node definition: moves made) on a
             # bounded sample of the same positions
@@ -193,7 +206,7 @@ def main():
            "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in kst.items() if v[0]},
            "roofline": roof, "cpu_baseline": cpu,
            "config1": {"workload": "startpos perft(5), one root", "leaves": c1, "ms": round(c1_ms, 3),
-                       "leaves_per_s": c1 / (c1_ms / 1e3)},
+                       "leaves_per_s": c1 / (c1_ms / 1e3), "cpu_baseline": c1_cpu},
            "search": search}
     if calib:
         out["calibration"] = calib
