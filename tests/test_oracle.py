@@ -148,8 +148,36 @@ ENDGAMES = [
 def test_search_equals_minimax(fen):
     # fail-hard alpha-beta with a full root window returns the minimax value
     p = R.setboard(fen)
-    for d in (1, 2):
+    for d in (1, 2, 3):
         assert R.search(p, d)[0] == R.minimax(p, d)[0]
+
+
+NO_CASTLE = 0x9100000000000091  # moved flags of a1, e1, h1, a8, e8, h8
+
+
+def _mirror_full(pos):
+    """Colour-flipped mirror image with the moved flags mirrored too and the
+    hash recomputed; castling rights are removed first (on both sides), since
+    the castle precondition masks d8 and not e8 (quirk Q2) and so is not
+    colour-symmetric."""
+    q = pos.copy()
+    q["b"][0, 4] = int(q["b"][0, 4]) | NO_CASTLE
+    m = _mirror(q)
+    m["b"][0, 4] = int.from_bytes(int(q["b"][0, 4]).to_bytes(8, "little")[::-1], "little")
+    for r in (q, m):
+        r["b"][0, 5] = R.lib().rc_computehash(r["b"][0].ctypes.data, int(r["stm"][0]))
+    return q, m
+
+
+def test_search_mirror_symmetric():
+    # a position and its colour-flipped mirror have the same value for the
+    # side to move and the same perft counts: a colour-dependent mistake in
+    # pawn directions, promotions, e.p. or the evaluation's sign breaks this
+    pos = R.gen_playouts(13, 24, 10, 60)
+    for i in range(len(pos)):
+        p, m = _mirror_full(pos[i:i + 1])
+        assert R.search(p, 4)[0] == R.search(m, 4)[0], i
+        assert R.perft(p, 3) == R.perft(m, 3), i
 
 
 def test_search_order_independent():
