@@ -4,7 +4,8 @@ numpy restatement of the network the GPU trains (908 -> 256 -> 32 -> 1,
 ReLU, pairwise ranking loss softplus(s_neg - s_pos)), in two forms:
   * forward_backward(..., bf16=True): FP32 arithmetic with BF16 rounding at
     the points where the GPU feeds BF16 MFMA operands (weights W1, W2,
-    activations h1, and the gradients dz2, dz1), the checker for the kernels;
+    activations h1, and the gradients dz2, dz1): the values the kernels'
+    results are compared with;
   * bf16=False in float64: the plain mathematics, pinned by finite
     differences in the tests.
 The network has no counterpart in the reference (whose move ordering is fixed
