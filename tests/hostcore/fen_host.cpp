@@ -37,3 +37,14 @@ extern "C" int fh_play(const char *fen, const char *moves, char *out, int cap) {
   std::memcpy(out, f.c_str(), f.size() + 1);
   return 0;
 }
+
+// Prints the board of `fen` as the front end's board print does (stdout).
+// Returns 0, or -1 for a FEN the parser rejects.
+extern "C" int fh_print(const char *fen) {
+  uint64_t b[7];
+  bool stm;
+  int64_t gp = 0;
+  if (!setboard(b, stm, fen, &gp)) return -1;
+  printboard(b, stm, gp, 0);
+  return 0;
+}

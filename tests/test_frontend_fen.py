@@ -30,12 +30,15 @@ def F():
     L = ctypes.CDLL(SO)
     L.fh_play.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     L.fh_play.restype = ctypes.c_int
+    L.fh_print.argtypes = [ctypes.c_char_p]
+    L.fh_print.restype = ctypes.c_int
 
     def play(fen, moves=""):
         out = ctypes.create_string_buffer(256)
         rc = L.fh_play(fen.encode(), moves.encode(), out, 256)
         assert rc == 0, (fen, moves, rc)
         return out.value.decode()
+    play.lib = L
     return play
 
 
@@ -89,3 +92,16 @@ def test_fen_ep_square_not_kept_from_input(F):
     out = ctypes.create_string_buffer(256)
     L = ctypes.CDLL(SO)
     assert L.fh_play(fen.encode(), b"e5d6", out, 256) == -2
+
+
+def test_board_print(F, capfd):
+    # the board print of src/io.c:164-225: files header, ranks 8..1 with "-"
+    # for an empty square, the header again, then the position's FEN
+    fen = "r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1"
+    capfd.readouterr()
+    assert F.lib.fh_print(fen.encode()) == 0
+    lines = capfd.readouterr().out.splitlines()
+    assert lines == ["###ABCDEFGH###",
+                     "#8 r---k--r", "#7 p-ppqpb-", "#6 bn--pnp-", "#5 ---PN---",
+                     "#4 -p--P---", "#3 --N--Q-p", "#2 PPPBBPPP", "#1 R---K--R",
+                     "###ABCDEFGH###", "#fen: " + fen]
