@@ -8,6 +8,15 @@
  * fixed-depth alpha-beta + quiescence node semantics.  Every function cites
  * the reference file:line it follows.
  *
+ * Pins (tests/test_oracle.py): the reference's 14 perft KATs
+ * (src/test.c:43-73), textbook perft counts, quirk positions, perft(d) as
+ * the sum over children; table invariants; evaluation closed forms and
+ * mirror antisymmetry; incremental hash = from-scratch hash; search =
+ * minimax on small trees, order independence, colour symmetry, terminal
+ * scores.  Parity unpinned: the evaluation tables (src/zeta.cl:272-359)
+ * beyond the entries the closed-form cases sum by hand -- no
+ * reference-held vector covers the evaluation or the search.
+ *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load this library, and only to check device results or as the timed CPU
  * baseline.  The product (libzetahip.so) never links, loads or calls it.
