@@ -3,6 +3,10 @@
 #include "../../zeta_amd/csrc/chess.h"
 #include "../../zeta_amd/csrc/evaltab.h"
 
+#include <atomic>
+#include <thread>
+#include <vector>
+
 using namespace zc;
 
 static Pos load(const uint64_t *b, int stm) {
@@ -152,4 +156,31 @@ extern "C" int32_t hc_eval(const uint64_t *b) {
   int32_t e = 0;
   for (int sq = 0; sq < 64; sq++) e += eval_square(p, sq);
   return e + eval_bishop_pair(p);
+}
+
+// The device perft algorithm on the host: interior levels expand every legal
+// move in the grouped order the expansion kernel writes (for_each_legal,
+// apply with the castle rook), the last level counts legal moves (bulk
+// counting, as the leaf kernel does).  One position per thread at a time.
+static uint64_t walk_bulk(const Pos &p, int d) {
+  Legal L;
+  analyse(p, L);
+  if (d == 1) return (uint64_t)count_legal(p, L, false);
+  uint64_t s = 0;
+  for_each_legal(p, L, [&](u32 code) { s += walk_bulk(apply(p, code, true), d - 1); });
+  return s;
+}
+
+extern "C" void hc_perft_batch(const uint64_t *pos, int n, int depth, uint64_t *out, int threads) {
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < (threads > 0 ? threads : 1); t++)
+    th.emplace_back([&] {
+      for (int i; (i = next++) < n;) {
+        const uint64_t *b = pos + (size_t)i * 8;  // 7 x u64 + stm (int32) + pad
+        const Pos p = load(b, (int)(uint32_t)b[7]);
+        out[i] = depth <= 0 ? 1 : walk_bulk(p, depth);
+      }
+    });
+  for (auto &t : th) t.join();
 }
