@@ -216,3 +216,46 @@ def test_device_perft_algorithm_reproduces_golden_workload(H):
     H.hc_perft_batch(pos.ctypes.data, len(pos), 4, out.ctypes.data, min(8, os.cpu_count() or 1))
     assert (out == gold["perft4"]).all()
     assert int(out.sum()) == 4182308444
+
+
+def _fen_with(piece, sq, wk, bk):
+    board = [["" for _ in range(8)] for _ in range(8)]
+    for ch, q in ((piece, sq), ("K", wk), ("k", bk)):
+        board[q // 8][q % 8] = ch
+    rows = []
+    for r in range(7, -1, -1):
+        row, run = "", 0
+        for f in range(8):
+            if board[r][f]:
+                row += (str(run) if run else "") + board[r][f]
+                run = 0
+            else:
+                run += 1
+        rows.append(row + (str(run) if run else ""))
+    return "/".join(rows) + " w - - 0 1"
+
+
+def test_device_eval_every_table_entry(H):
+    # one position per (piece, colour, square) beside the two kings: every
+    # entry of the device's evaluation tables is compared with the oracle's
+    # separately typed tables
+    n = 0
+    for piece in "PNBRQpnbrq":
+        for sq in range(64):
+            if piece in "Pp" and sq // 8 in (0, 7):
+                continue
+            wk, bk = (0, 63) if sq not in (0, 63) else (7, 56)
+            p = R.setboard(_fen_with(piece, sq, wk, bk))
+            b = np.ascontiguousarray(p[0]["b"])
+            assert H.hc_eval(b.ctypes.data) == R.evaluate(p), (piece, sq)
+            n += 1
+    for sq in range(64):  # the king tables, the other king kept away
+        for wk_side in (True, False):
+            other = 63 if sq < 32 else 0
+            fen = _fen_with("N", 27 if sq not in (27,) and other != 27 else 36, sq if wk_side else other,
+                            other if wk_side else sq)
+            p = R.setboard(fen)
+            b = np.ascontiguousarray(p[0]["b"])
+            assert H.hc_eval(b.ctypes.data) == R.evaluate(p), (sq, wk_side)
+            n += 1
+    assert n == 2 * (4 * 64 + 48) + 128
