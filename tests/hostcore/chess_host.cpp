@@ -184,3 +184,6 @@ extern "C" void hc_perft_batch(const uint64_t *pos, int n, int depth, uint64_t *
     });
   for (auto &t : th) t.join();
 }
+
+// Zobrist hash of a position from scratch, as the device computes it
+extern "C" uint64_t hc_hash(const uint64_t *b, int stm) { return hash_full(load(b, stm)); }

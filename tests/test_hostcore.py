@@ -41,6 +41,8 @@ def H():
     L.hc_slider_ks.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
     L.hc_slider_ks.restype = ctypes.c_uint64
     L.hc_perft_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
+    L.hc_hash.argtypes = [vp, ctypes.c_int]
+    L.hc_hash.restype = ctypes.c_uint64
     return L
 
 
@@ -259,3 +261,23 @@ def test_device_eval_every_table_entry(H):
             assert H.hc_eval(b.ctypes.data) == R.evaluate(p), (sq, wk_side)
             n += 1
     assert n == 2 * (4 * 64 + 48) + 128
+
+
+def test_device_hash_matches_oracle(H):
+    # the device's from-scratch Zobrist hash (its own copy of the keys) = the
+    # oracle's computehash, on playout positions and on every position one
+    # ply further (double pushes leave an e.p. flag, king and rook moves
+    # clear castle rights)
+    base = np.concatenate([R.gen_playouts(2, 300, 8, 40), R.gen_playouts(3, 300, 20, 60),
+                           R.setboard(R.STARTPOS),
+                           R.setboard("r3k2r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQkq - 0 1")])
+    n = 0
+    for i in range(len(base)):
+        p = base[i:i + 1]
+        kids = [p] + [R.domove(p, R.nth_legal(p, k)) for k in range(R.count_legal(p))]
+        for q in kids:
+            b = np.ascontiguousarray(q[0]["b"])
+            want = R.lib().rc_computehash(b.ctypes.data, int(q["stm"][0]))
+            assert H.hc_hash(b.ctypes.data, int(q["stm"][0])) == want, i
+            n += 1
+    assert n > 10000
