@@ -48,3 +48,22 @@ extern "C" int fh_print(const char *fen) {
   printboard(b, stm, gp, 0);
   return 0;
 }
+
+// Position i of the front end's SMP benchmark sets (suite 0: benchkaufmann,
+// 1: benchhyatt24), written back as a FEN; the number of legal moves in
+// *nmoves.  Returns 0, -1 for a FEN the parser rejects, -2 for an index out of
+// range, -3 for an output buffer that is too small.
+extern "C" int fh_suite(int suite, int i, char *out, int cap, int *nmoves) {
+  const char *const *set = suite ? HYATT24 : KAUFMANN;
+  const int n = suite ? 24 : 25;
+  if (i < 0 || i >= n) return -2;
+  uint64_t b[7];
+  bool stm;
+  int64_t gp = 0;
+  if (!setboard(b, stm, set[i], &gp)) return -1;
+  *nmoves = (int)legal_moves(b, stm).size();
+  const std::string f = createfen(b, stm, gp, 0);
+  if ((int)f.size() + 1 > cap) return -3;
+  std::memcpy(out, f.c_str(), f.size() + 1);
+  return 0;
+}

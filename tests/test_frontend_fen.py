@@ -32,6 +32,8 @@ def F():
     L.fh_play.restype = ctypes.c_int
     L.fh_print.argtypes = [ctypes.c_char_p]
     L.fh_print.restype = ctypes.c_int
+    L.fh_suite.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.fh_suite.restype = ctypes.c_int
 
     def play(fen, moves=""):
         out = ctypes.create_string_buffer(256)
@@ -105,3 +107,22 @@ def test_board_print(F, capfd):
                      "#8 r---k--r", "#7 p-ppqpb-", "#6 bn--pnp-", "#5 ---PN---",
                      "#4 -p--P---", "#3 --N--Q-p", "#2 PPPBBPPP", "#1 R---K--R",
                      "###ABCDEFGH###", "#fen: " + fen]
+
+
+def test_smp_benchmark_position_sets(F):
+    # benchkaufmann's 25 and benchhyatt24's 24 positions (src/xboard.c:986-1013,
+    # 1214-1240) all parse, read back to the same board, side to move and
+    # castle rights, and have legal moves
+    L = F.lib
+    for suite, n, first in ((0, 25, "1rbq1rk1/p1b1nppp/1p2p3/8/1B1pN3/P2B4/1P3PPP/2RQ1R1K w - -"),
+                            (1, 24, "r2qkbnr/ppp2p1p/2n5/3P4/2BP1pb1/2N2p2/PPPQ2PP/R1B2RK1 b kq -")):
+        out = ctypes.create_string_buffer(256)
+        nm = ctypes.c_int(0)
+        fens = []
+        for i in range(n):
+            assert L.fh_suite(suite, i, out, 256, ctypes.byref(nm)) == 0, (suite, i)
+            assert nm.value > 0, (suite, i)
+            fens.append(out.value.decode())
+        assert L.fh_suite(suite, n, out, 256, ctypes.byref(nm)) == -2
+        assert fens[0].rsplit(" ", 2)[0] == first
+        assert len(set(fens)) == n

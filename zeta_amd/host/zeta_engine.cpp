@@ -627,6 +627,108 @@ bool guess_config(bool extended) {
   return true;
 }
 
+// The two position sets of the reference's SMP benchmarks (the same
+// positions, so that the commands measure the same thing):
+// benchkaufmann, src/xboard.c:986-1013
+const char *const KAUFMANN[25] = {
+    "1rbq1rk1/p1b1nppp/1p2p3/8/1B1pN3/P2B4/1P3PPP/2RQ1R1K w - -",
+    "3r2k1/p2r1p1p/1p2p1p1/q4n2/3P4/PQ5P/1P1RNPP1/3R2K1 b - -",
+    "3r2k1/1p3ppp/2pq4/p1n5/P6P/1P6/1PB2QP1/1K2R3 w - -",
+    "r1b1r1k1/1ppn1p1p/3pnqp1/8/p1P1P3/5P2/PbNQNBPP/1R2RB1K w - -",
+    "2r4k/pB4bp/1p4p1/6q1/1P1n4/2N5/P4PPP/2R1Q1K1 b - -",
+    "r5k1/3n1ppp/1p6/3p1p2/3P1B2/r3P2P/PR3PP1/2R3K1 b - -",
+    "2r2rk1/1bqnbpp1/1p1ppn1p/pP6/N1P1P3/P2B1N1P/1B2QPP1/R2R2K1 b - -",
+    "5r1k/6pp/1n2Q3/4p3/8/7P/PP4PK/R1B1q3 b - -",
+    "r3k2r/pbn2ppp/8/1P1pP3/P1qP4/5B2/3Q1PPP/R3K2R w KQkq -",
+    "3r2k1/ppq2pp1/4p2p/3n3P/3N2P1/2P5/PP2QP2/K2R4 b - -",
+    "q3rn1k/2QR4/pp2pp2/8/P1P5/1P4N1/6n1/6K1 w - -",
+    "6k1/p3q2p/1nr3pB/8/3Q1P2/6P1/PP5P/3R2K1 b - -",
+    "1r4k1/7p/5np1/3p3n/8/2NB4/7P/3N1RK1 w - -",
+    "1r2r1k1/p4p1p/6pB/q7/8/3Q2P1/PbP2PKP/1R3R2 w - -",
+    "r2q1r1k/pb3p1p/2n1p2Q/5p2/8/3B2N1/PP3PPP/R3R1K1 w - -",
+    "8/4p3/p2p4/2pP4/2P1P3/1P4k1/1P1K4/8 w - -",
+    "1r1q1rk1/p1p2pbp/2pp1np1/6B1/4P3/2NQ4/PPP2PPP/3R1RK1 w - -",
+    "q4rk1/1n1Qbppp/2p5/1p2p3/1P2P3/2P4P/6P1/2B1NRK1 b - -",
+    "r2q1r1k/1b1nN2p/pp3pp1/8/Q7/PP5P/1BP2RPN/7K w - -",
+    "8/5p2/pk2p3/4P2p/2b1pP1P/P3P2B/8/7K w - -",
+    "8/2k5/4p3/1nb2p2/2K5/8/6B1/8 w - -",
+    "1B1b4/7K/1p6/1k6/8/8/8/8 w - -",
+    "rn1q1rk1/1b2bppp/1pn1p3/p2pP3/3P4/P2BBN1P/1P1N1PP1/R2Q1RK1 b - -",
+    "8/p1ppk1p1/2n2p2/8/4B3/2P1KPP1/1P5P/8 w - -",
+    "8/3nk3/3pp3/1B6/8/3PPP2/4K3/8 w - -"};
+// benchhyatt24, src/xboard.c:1214-1240
+const char *const HYATT24[24] = {
+    "r2qkbnr/ppp2p1p/2n5/3P4/2BP1pb1/2N2p2/PPPQ2PP/R1B2RK1 b kq -",
+    "r2qkbnr/ppp2p1p/8/nB1P4/3P1pb1/2N2p2/PPPQ2PP/R1B2RK1 b kq -",
+    "r2qkbnr/pp3p1p/2p5/nB1P4/3P1Qb1/2N2p2/PPP3PP/R1B2RK1 b kq -",
+    "r2qkb1r/pp3p1p/2p2n2/nB1P4/3P1Qb1/2N2p2/PPP3PP/R1B1R1K1 b kq -",
+    "r2q1b1r/pp1k1p1p/2P2n2/nB6/3P1Qb1/2N2p2/PPP3PP/R1B1R1K1 b - -",
+    "r2q1b1r/p2k1p1p/2p2n2/nB6/3PNQb1/5p2/PPP3PP/R1B1R1K1 b - -",
+    "r2q1b1r/p2k1p1p/2p5/nB6/3Pn1Q1/5p2/PPP3PP/R1B1R1K1 b - -",
+    "r2q1b1r/p1k2p1p/2p5/nB6/3PR1Q1/5p2/PPP3PP/R1B3K1 b - -",
+    "r2q1b1r/p1k2p1p/8/np6/3PR3/5Q2/PPP3PP/R1B3K1 b - -",
+    "r4b1r/p1kq1p1p/8/np6/3P1R2/5Q2/PPP3PP/R1B3K1 b - -",
+    "r6r/p1kqbR1p/8/np6/3P4/5Q2/PPP3PP/R1B3K1 b - -",
+    "5r1r/p1kqbR1p/8/np6/3P1B2/5Q2/PPP3PP/R5K1 b - -",
+    "5r1r/p2qbR1p/1k6/np2B3/3P4/5Q2/PPP3PP/R5K1 b - -",
+    "5rr1/p2qbR1p/1k6/np2B3/3P4/2P2Q2/PP4PP/R5K1 b - -",
+    "5rr1/p2qbR1p/1kn5/1p2B3/3P4/2P2Q2/PP4PP/4R1K1 b - -",
+    "4qRr1/p3b2p/1kn5/1p2B3/3P4/2P2Q2/PP4PP/4R1K1 b - -",
+    "5qr1/p3b2p/1kn5/1p1QB3/3P4/2P5/PP4PP/4R1K1 b - -",
+    "5q2/p3b2p/1kn5/1p1QB1r1/P2P4/2P5/1P4PP/4R1K1 b - -",
+    "5q2/p3b2p/1kn5/3QB1r1/p1PP4/8/1P4PP/4R1K1 b - -",
+    "5q2/p3b2p/1k6/3QR1r1/p1PP4/8/1P4PP/6K1 b - -",
+    "5q2/p3b2p/1k6/4Q3/p1PP4/8/1P4PP/6K1 b - -",
+    "3q4/p3b2p/1k6/2P1Q3/p2P4/8/1P4PP/6K1 b - -",
+    "3q4/p3b2p/8/1kP5/p2P4/8/1P2Q1PP/6K1 b - -",
+    "3q4/p3b2p/8/2P5/pk1P4/3Q4/1P4PP/6K1 b - -"};
+
+// SMP benchmark over a position set (src/xboard.c:975-1201 and 1203-1427):
+// every position is searched to depth sd with 1, 2, 4, ... workers up to the
+// configured count; a table per position, then the time-to-depth speedups
+// over the summed times of all positions.
+void bench_suite(const char *name, const char *const *fens, int npos) {
+  const uint64_t wmax = totalWorkUnits;
+  std::vector<double> tall;
+  std::vector<uint64_t> wall;
+  for (int posi = 0; posi < npos; posi++) {
+    out("#\n#\n### setting up board %d: ###\n", posi + 1);
+    new_game(fens[posi]);
+    printboard(BOARD, STM, GAMEPLY, PLY);
+    std::vector<double> t;
+    std::vector<uint64_t> nps, wk;
+    for (uint64_t w = 1;; w *= 2) {
+      if (w > wmax) w = wmax;
+      out("### doing inits for %s depth %d: ###\n", name, SD);
+      totalWorkUnits = w;
+      if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+      out("### computing %s depth %d: ###\n", name, SD);
+      out("### work-groups: %" PRIu64 " ###\n", w);
+      const double start = now_ms();
+      rootsearch(BOARD, STM, SD);
+      const double el = (now_ms() - start + 1) / 1000;
+      t.push_back(el);
+      nps.push_back((uint64_t)((double)ABNODECOUNT / el));
+      wk.push_back(w);
+      if (w == wmax) break;
+    }
+    out("### workers\t#nps\t\t#nps speedup\t#time in s\t#ttd speedup \t#relative ttd speedup ###\n");
+    for (size_t i = 0; i < t.size(); i++) {
+      const double npsx = i ? (double)nps[i] / (double)(nps[0] ? nps[0] : 1) : 1.0;
+      out("### %" PRIu64 "\t\t%" PRIu64 "\t\t%lf\t%lf\t%lf\t%lf \n", wk[i], nps[i], npsx, t[i], t[0] / t[i],
+          i ? t[i - 1] / t[i] : 1.0);
+    }
+    if (tall.size() < t.size()) tall.resize(t.size(), 0.0);
+    for (size_t i = 0; i < t.size(); i++) tall[i] += t[i];
+    wall = wk;
+  }
+  out("#\n# overall results\n### workers\t#ttd speedup\t#rel ttd speedup ###\n");
+  for (size_t i = 0; i < tall.size(); i++)
+    out("### %" PRIu64 "\t\t%lf\t%lf \n", wall[i], tall[0] / tall[i], i ? tall[i - 1] / tall[i] : 1.0);
+  totalWorkUnits = wmax;
+  if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+}
+
 void print_help() {
   out("Zeta %s, MI355X build. Options:\n", VERSION);
   out(" -h, --help         Print help\n -v, --version      Print version\n");
@@ -634,6 +736,8 @@ void print_help() {
   out(" --guessconfig      Measure the device and write config.txt (quick)\n");
   out(" --guessconfigx     Measure the device and write config.txt (longer)\n");
   out(" --config FILE      Read FILE instead of ./config.txt\n");
+  out("Commands outside xboard mode: sd N, setboard FEN, go, perft, selftest, bench, benchsmp,\n"
+      " benchkaufmann, benchhyatt24 (search depth N with 1, 2, 4, ... workers), help, quit\n");
 }
 
 // --------------------------------------------------------------- xboard
@@ -775,6 +879,10 @@ void xboard() {
       out("#\n# overall results\n### workers\t#ttd speedup\t#rel ttd speedup ###\n");
       for (size_t i = 0; i < secs.size(); i++)
         out("### %" PRIu64 "\t\t%lf\t%lf \n", ws[i], secs[0] / secs[i], i ? secs[i - 1] / secs[i] : 1.0);
+    } else if (!xboard_mode && !strcmp(cmd, "benchkaufmann")) {
+      bench_suite("benchkaufmann", KAUFMANN, 25);
+    } else if (!xboard_mode && !strcmp(cmd, "benchhyatt24")) {
+      bench_suite("benchhyatt24", HYATT24, 24);
     } else if (!xboard_mode && !strcmp(cmd, "bench")) {
       // fixed-depth search of the current position (time to depth and nps)
       const double start = now_ms();
