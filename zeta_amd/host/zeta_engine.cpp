@@ -902,6 +902,12 @@ void xboard() {
         fclose(LogFile);
         LogFile = nullptr;
       }
+    } else if (!strcmp(cmd, "analyze") || !strcmp(cmd, "pause") || !strcmp(cmd, "resume") ||
+               !strcmp(cmd, "edit")) {
+      // named but unsupported commands also tell the user (src/xboard.c:1464-1490)
+      out("Error (unsupported command): %s\n", cmd);
+      out("tellusererror (unsupported command): %s\n", cmd);
+      if (!strcmp(cmd, "edit")) out("tellusererror engine supports only CECP (Xboard) version >=2\n");
     } else {
       out("Error (unsupported command): %s\n", cmd);
     }
