@@ -192,6 +192,12 @@ def test_search_terminal_scores():
     INF = 32000
     # mate in one (back rank): root score INF - 1 (child mated at ply 1)
     assert R.search(R.setboard("6k1/5ppp/8/8/8/8/5PPP/R5K1 w - - 0 1"), 2)[0] == INF - 1
+    # mate in two (1.Re8+ Rxe8 2.Rxe8#): INF - 3, for either colour, and
+    # minimax agrees; the side being mated, one reply before: -(INF - 2)
+    for fen in ("r5k1/5ppp/8/8/8/8/4RPPP/4R1K1 w - - 0 1", "4r1k1/4rppp/8/8/8/8/5PPP/R5K1 b - - 0 1"):
+        p = R.setboard(fen)
+        assert R.search(p, 3)[0] == INF - 3 == R.minimax(p, 3)[0]
+    assert R.search(R.setboard("r3R1k1/5ppp/8/8/8/8/5PPP/4R1K1 b - - 1 1"), 3)[0] == -(INF - 2)
     # mated root / stalemate root: PV[0] stays DRAWSCORE
     assert R.search(R.setboard("R5k1/5ppp/8/8/8/8/5PPP/6K1 b - - 0 1"), 3)[0] == 0
     assert R.search(R.setboard("7k/5Q2/6K1/8/8/8/8/8 b - - 0 1"), 3)[0] == 0
