@@ -589,27 +589,38 @@ bool guess_config(bool extended) {
     out("Error (%s): --guessconfig\n", msg);
     return false;
   }
-  const int npos = extended ? 64 : 16;
+  // the engine searches one root at a time: each position is searched alone
+  // (ID 1..depth, TT cleared), and the worker count with the shortest total
+  // time to depth wins (more workers always raise nodes/s, not always the
+  // time to depth)
+  const int npos = extended ? 16 : 4;
   std::vector<zeta_pos> pos(npos);
   rc = zeta_playouts(3, npos, 20, 60, pos.data());
   std::vector<zeta_result> res(npos);
   int best_w = 1;
-  double best_nps = 0;
-  const int cands[] = {1, 4, 16, 64};
+  double best_t = 0, best_nps = 0;
+  const std::vector<int> cands = extended ? std::vector<int>{1, 2, 4, 8, 16, 32, 64} : std::vector<int>{1, 4, 16, 64};
   for (int w : cands) {
     zeta_search_opts o;
     memset(&o, 0, sizeof o);
-    o.depth = extended ? 5 : 4;
+    o.depth = extended ? 6 : 5;
     o.workers = w;
     o.tt_log2 = 24;
     o.clear_tt = 1;
-    const double t0 = now_ms();
-    rc = rc ? rc : zeta_search_batch(pos.data(), 1, &o, res.data());  // one root: the engine's case
-    const double dt = (now_ms() - t0) / 1000;
+    double total = 0;
+    uint64_t nodes = 0;
+    for (int i = 0; i < npos && !rc; i++) {
+      const double t0 = now_ms();
+      rc = zeta_search_batch(&pos[i], 1, &o, &res[i]);
+      total += (now_ms() - t0) / 1000;
+      nodes += res[i].nodes;
+    }
     if (rc) break;
-    const double nps = res[0].nodes / (dt > 0 ? dt : 1e-9);
-    out("#> workers per root %d: %.0f nodes/s\n", w, nps);
-    if (nps > best_nps) {
+    const double nps = (double)nodes / (total > 0 ? total : 1e-9);
+    out("#> workers per root %d: time to depth %d over %d positions %.3f s, %.0f nodes/s\n", w, o.depth, npos,
+        total, nps);
+    if (best_t == 0 || total < best_t) {
+      best_t = total;
       best_nps = nps;
       best_w = w;
     }
