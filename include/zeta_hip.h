@@ -219,7 +219,9 @@ int zeta_records_put(const void *in, uint64_t n);
 /* ------------------------------------------------------------ device memory
  * Device buffers for callers that keep inputs resident on the GPU (the
  * *_device entry points).  Plain pointers and sizes; all copies are
- * synchronous with respect to the library's stream. */
+ * synchronous with respect to the library's stream.  No counterpart at the
+ * reference's boundary: its buffers stay inside src/clrun.c
+ * (clCreateBuffer / clEnqueueReadBuffer, src/clrun.c:279-1225). */
 int zeta_dev_alloc(void **dptr, size_t bytes);
 int zeta_dev_free(void *dptr);
 int zeta_memcpy_htod(void *dst, const void *src, size_t bytes);
@@ -231,7 +233,9 @@ int zeta_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int zeta_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int zeta_synchronize(void);
 
-/* ------------------------------------------------------------------ stats */
+/* ------------------------------------------------------------------ stats
+ * Per-kernel HIP-event times.  No counterpart in the reference, which times
+ * whole runs on the host (src/search.c:48-123, 243). */
 #define ZETA_MAX_KSTATS 16
 typedef struct {
   char name[32];
