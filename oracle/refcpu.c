@@ -177,6 +177,20 @@ void rc_init(void) {
   g_inited = 1;
 }
 
+/* constants as the oracle holds them, for the test that pins them to numbers
+ * extracted from the reference: which 0 EVAL_VALUE, 1 EVAL_CONTROL,
+ * 2 EVAL_PST (src/zeta.cl:272-359); keys src/zobrist.h:28-35; masks
+ * src/types.h:114-124 in the order SMCRALL, WHITEQ, WHITEK, BLACKQ, BLACKK */
+int32_t rc_eval_const(int which, int idx) {
+  if (which == 0) return (idx >= 0 && idx < 7) ? EVAL_VALUE[idx] : 0;
+  if (which == 1) return (idx >= 0 && idx < 64) ? EVAL_CONTROL[idx] : 0;
+  return (idx >= 0 && idx < 7 * 64) ? EVAL_PST[idx] : 0;
+}
+uint64_t rc_zobrist_key(int i) { return (i >= 0 && i < 18) ? ZK[i] : 0; }
+uint64_t rc_castle_mask(int i) {
+  static const u64 m[5] = {SMCRALL, SMCRWHITEQ, SMCRWHITEK, SMCRBLACKQ, SMCRBLACKK};
+  return (i >= 0 && i < 5) ? m[i] : 0;
+}
 uint64_t rc_attack_table(int idx) { rc_init(); return AT[idx]; }
 uint64_t rc_pawnpush_table(int idx) { rc_init(); return PP[idx]; }
 uint64_t rc_inbetween(int a, int b) { rc_init(); return IB[a * 64 + b]; }

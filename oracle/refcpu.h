@@ -13,9 +13,10 @@
  * the sum over children; table invariants; evaluation closed forms and
  * mirror antisymmetry; incremental hash = from-scratch hash; search =
  * minimax on small trees, order independence, colour symmetry, terminal
- * scores.  Parity unpinned: the evaluation tables (src/zeta.cl:272-359)
- * beyond the entries the closed-form cases sum by hand -- no
- * reference-held vector covers the evaluation or the search.
+ * scores.  The evaluation tables, Zobrist keys and castle masks are pinned
+ * entry by entry to numbers extracted from the reference
+ * (tests/golden/ref_tables.json, tests/test_tables_pinned.py).  No
+ * reference-held vector covers the search itself.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load this library, and only to check device results or as the timed CPU
@@ -58,6 +59,12 @@ uint64_t rc_pawnpush_table(int idx);    /* AttackTablesPawnPushes[128] src/zetap
 uint64_t rc_inbetween(int a, int b);    /* bbInBetween[64*64]        src/bitboard.c:52-118 */
 uint64_t rc_line(int a, int b);         /* bbLine[64*64]             src/bitboard.c:119-185 */
 uint64_t rc_sqinbetween(int a, int b);  /* closed form, src/bit.c:49-66 */
+
+/* constants, for the test that pins them to the reference's numbers:
+ * which 0 material[7], 1 square control[64], 2 piece-square[7*64] */
+int32_t  rc_eval_const(int which, int idx);
+uint64_t rc_zobrist_key(int i);         /* Zobrist[18]   src/zobrist.h:28-35 */
+uint64_t rc_castle_mask(int i);         /* SMCRALL, SMCRWHITEQ/K, SMCRBLACKQ/K  src/types.h:114-124 */
 
 /* ---- move generation ---- */
 /* Per-square legal move bitboards exactly as the x64 generator leaves them in

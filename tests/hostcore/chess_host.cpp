@@ -187,3 +187,19 @@ extern "C" void hc_perft_batch(const uint64_t *pos, int n, int depth, uint64_t *
 
 // Zobrist hash of a position from scratch, as the device computes it
 extern "C" uint64_t hc_hash(const uint64_t *b, int stm) { return hash_full(load(b, stm)); }
+
+// The device code's constants, for the test that pins them to the numbers
+// extracted from the reference: which 0 EVAL_VALUE, 1 EVAL_CONTROL, 2 EVAL_PST,
+// 3 the folded per-square table the evaluation reads (10 + material +
+// piece-square + control)
+extern "C" int32_t hc_eval_const(int which, int idx) {
+  if (which == 0) return (idx >= 0 && idx < 7) ? EVAL_VALUE[idx] : 0;
+  if (which == 1) return (idx >= 0 && idx < 64) ? EVAL_CONTROL[idx] : 0;
+  if (which == 2) return (idx >= 0 && idx < 7 * 64) ? EVAL_PST[idx] : 0;
+  return (idx >= 0 && idx < 7 * 64) ? EVAL_SQ.v[idx] : 0;
+}
+extern "C" uint64_t hc_zobrist_key(int i) { return (i >= 0 && i < 18) ? ZOBRIST[i] : 0; }
+extern "C" uint64_t hc_castle_mask(int i) {
+  const u64 m[5] = {SMCRALL, SMCRWHITEQ, SMCRWHITEK, SMCRBLACKQ, SMCRBLACKK};
+  return (i >= 0 && i < 5) ? m[i] : 0;
+}
