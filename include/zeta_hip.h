@@ -28,7 +28,8 @@ extern "C" {
 #define ZETA_ENOMEM (-4)   /* device allocation failed or budget exceeded */
 #define ZETA_ESTATE (-5)   /* library not initialised / wrong state */
 #define ZETA_EBUSY (-6)    /* not admitted now (serving slots / memory budget), or the device is serving */
-#define ZETA_ECANCELED (-7) /* serving request cancelled (zeta_serve_cancel / stop / watchdog) */
+#define ZETA_ECANCELED (-7) /* serving request cancelled (zeta_serve_cancel / stop / watchdog), or a search root stopped by zeta_search_abort */
+#define ZETA_EBUDGET (-8)   /* search root stopped at its node budget (zeta_search_opts.max_nodes) */
 #define ZETA_PENDING 1      /* zeta_serve_poll: not finished yet */
 
 /* quad-bitboard slots (src/types.h:60-66) */
@@ -96,7 +97,8 @@ typedef struct {
   int workers;        /* wavefront workers per root at most (ABDADA helpers), 1..1024 */
   int tt_log2;        /* TT size 2^tt_log2 x 16 B; -1 = zeta_config.tt_log2; 0 = none */
   int clear_tt;       /* 1 = clear the TT before the batch */
-  uint64_t max_nodes; /* per-worker node budget per root, 0 = none */
+  uint64_t max_nodes; /* per-worker node budget per root, 0 = none: the first worker over it
+                         stops the root (src/zeta.cl:1945-1947); status ZETA_EBUDGET */
   int use_mlp;        /* learned move ordering at interior nodes (needs zeta_mlp_load) */
   int mlp_min_depth;  /* nodes with depth >= this use the learned ordering */
   int log_records;    /* record cutoff examples for training (zeta_records_*) */
@@ -121,10 +123,23 @@ typedef struct {
   uint32_t bestmove; /* reference move code (src/types.h:67-74) */
   uint64_t nodes;    /* moves made by all workers on this root */
   int32_t depth;     /* deepest completed iteration */
-  int32_t status;    /* 0 ok, else a ZETA_E* code */
+  int32_t status;    /* 0 ok; ZETA_EBUDGET: stopped at the node budget, score and bestmove are
+                        those of the deepest iteration the stopping worker completed (0 if none;
+                        the reference's host discards such a search, src/search.c:133);
+                        ZETA_ECANCELED: stopped by zeta_search_abort; ZETA_ENOMEM: move
+                        stack overflow */
 } zeta_result;
 
 int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_opts *opts, zeta_result *res);
+/* Abort switch of the search batches.  No counterpart at the reference's
+ * boundary (its helpers stop on the kernel's own finito flag and the node
+ * budget, src/zeta.cl:1940-1953).  on = 1: a zeta_search_batch running on
+ * another host thread stops at its workers' next poll (every 64 nodes), and
+ * every batch started while the switch is on returns at once; the roots left
+ * unfinished get status ZETA_ECANCELED.  on = 0 re-arms.  The one call that
+ * may be made while zeta_search_batch runs; it touches a word in pinned host
+ * memory only. */
+int zeta_search_abort(int on);
 /* Counters of the last zeta_search_batch: [0] TT probes, [1] TT key hits,
  * [2] TT scores used, [3] TT stores, [4] ABDADA deferrals, [5] MAXPLY cuts,
  * [6] nodes ordered by the network, [7] training pairs logged, [8] nodes

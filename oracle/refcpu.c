@@ -664,10 +664,19 @@ typedef struct {
   u32 bestmove;
   int minimax;
   int reverse; /* full-width nodes take moves worst-first (order-independence checks) */
+  /* the reference's quiet-move heuristics (src/zeta.cl:1916-1933, 2011-2082):
+   * one killer per ply and a counter move per previous move's squares; kept
+   * across the iterations of one root */
+  u32 killer[RC_MAXPLY + 2];
+  u32 counter[64 * 64];
 } sctx;
 
 /* move ordering score of the reference picker (src/zeta.cl:2059-2071); any
- * order gives the same root score, this one only keeps the oracle fast. */
+ * order gives the same root score.  The search below orders by it, then the
+ * counter move (1300) and the killer (1400) over it, highest first and the
+ * earliest generated among equals -- the device search's order, so a
+ * single-worker device search without a transposition table visits the same
+ * tree and makes the same number of moves. */
 static s32 order_score(const u64 *b, int stm, u32 move) {
   int lid = GETSQFROM(move), sqto = GETSQTO(move);
   u32 pf = GETPFROM(move), pt = GETPTO(move), pc = GETPCPT(move);
@@ -716,7 +725,10 @@ static s32 ab(sctx *c, const u64 *b, u64 cr, u64 hash, u8 hmc, int stm, int dept
 
   u32 moves[256];
   s32 scores[256];
+  unsigned char used[256];
   int n = 0;
+  const u32 killer = c->killer[sd];
+  const u32 counter = c->counter[GETSQFROM(lastmove) * 64 + GETSQTO(lastmove)];
   for (int lid = 0; lid < 64; lid++) {
     u64 m = mv[lid];
     while (m) {
@@ -724,17 +736,23 @@ static s32 ab(sctx *c, const u64 *b, u64 cr, u64 hash, u8 hmc, int stm, int dept
       u32 move = rc_makemove_code(b, stm, lid, sqto);
       moves[n] = move;
       scores[n] = order_score(b, stm, move);
+      if (!c->minimax && !c->reverse && GETPTYPE(GETPCPT(move)) == PNONE) {
+        if (move == killer) scores[n] = 1400;        /* src/zeta.cl:2078-2082 */
+        else if (move == counter) scores[n] = 1300;  /* src/zeta.cl:2072-2076 */
+      }
+      used[n] = 0;
       n++;
     }
   }
   s32 mmbest = -RC_INF - 1;
   for (int i = 0; i < n; i++) {
-    int bi = i;
-    for (int j = i + 1; j < n; j++)
-      if ((c->reverse && !qs) ? (scores[j] < scores[bi]) : (scores[j] > scores[bi])) bi = j;
-    u32 mt = moves[i]; moves[i] = moves[bi]; moves[bi] = mt;
-    s32 st = scores[i]; scores[i] = scores[bi]; scores[bi] = st;
-    u32 move = moves[i];
+    int bi = -1;
+    for (int j = 0; j < n; j++) {
+      if (used[j]) continue;
+      if (bi < 0 || ((c->reverse && !qs) ? (scores[j] < scores[bi]) : (scores[j] > scores[bi]))) bi = j;
+    }
+    used[bi] = 1;
+    u32 move = moves[bi];
     u64 child[4] = {b[0], b[1], b[2], b[3]};
     domove4(child, move);
     u64 ccr = cr_after_ab(cr, move);
@@ -757,14 +775,22 @@ static s32 ab(sctx *c, const u64 *b, u64 cr, u64 hash, u8 hmc, int stm, int dept
       alpha = s;
       if (sd == 1) { c->bestscore = s; c->bestmove = move; }
     }
-    if (alpha >= beta) break;
+    if (alpha >= beta) {
+      /* a quiet move failing high outside quiescence becomes the killer of
+       * the ply and the counter move of the move before (src/zeta.cl:1916-1933) */
+      if (!qs && GETPTYPE(GETPCPT(move)) == PNONE) {
+        c->killer[sd] = move;
+        c->counter[GETSQFROM(lastmove) * 64 + GETSQTO(lastmove)] = move;
+      }
+      break;
+    }
   }
   if (c->minimax) return (best_standpat > mmbest) ? best_standpat : mmbest;
   return alpha;
 }
 
 static s32 search_common(const rc_pos *p, int depth, const uint64_t *hist, int ply_init, uint64_t *nodes,
-                         uint32_t *bestmove, int minimax, int reverse) {
+                         uint32_t *bestmove, int minimax, int reverse, int id) {
   rc_init();
   sctx *c = (sctx *)calloc(1, sizeof(sctx));
   if (!c) return 0;
@@ -778,8 +804,12 @@ static s32 search_common(const rc_pos *p, int depth, const uint64_t *hist, int p
   }
   c->minimax = minimax;
   c->reverse = reverse;
-  c->bestscore = 0; /* DRAWSCORE until a root move raises alpha (src/zeta.cl:986) */
-  ab(c, p->b, p->b[4], p->b[5], (u8)p->b[6], p->stm, depth, -RC_INF, RC_INF, 1, 0, 0);
+  /* iterative deepening 1..depth (src/search.c:77-198) with the killers and
+   * counter moves kept across iterations; or the one iteration at depth */
+  for (int d = id ? 1 : depth; d <= depth; d++) {
+    c->bestscore = 0; /* DRAWSCORE until a root move raises alpha (src/zeta.cl:986) */
+    ab(c, p->b, p->b[4], p->b[5], (u8)p->b[6], p->stm, d, -RC_INF, RC_INF, 1, 0, 0);
+  }
   s32 sc = c->bestscore;
   if (sc == RC_INF || sc == -RC_INF) sc = 0; /* src/search.c:137 */
   if (nodes) *nodes = c->nodes;
@@ -790,13 +820,17 @@ static s32 search_common(const rc_pos *p, int depth, const uint64_t *hist, int p
 
 int32_t rc_search(const rc_pos *p, int depth, const uint64_t *hist, int ply_init, uint64_t *nodes,
                   uint32_t *bestmove) {
-  return search_common(p, depth, hist, ply_init, nodes, bestmove, 0, 0);
+  return search_common(p, depth, hist, ply_init, nodes, bestmove, 0, 0, 0);
+}
+int32_t rc_search_id(const rc_pos *p, int depth, const uint64_t *hist, int ply_init, uint64_t *nodes,
+                     uint32_t *bestmove) {
+  return search_common(p, depth, hist, ply_init, nodes, bestmove, 0, 0, 1);
 }
 int32_t rc_minimax(const rc_pos *p, int depth, uint64_t *nodes) {
-  return search_common(p, depth, NULL, 0, nodes, NULL, 1, 0);
+  return search_common(p, depth, NULL, 0, nodes, NULL, 1, 0, 0);
 }
 int32_t rc_search_reversed(const rc_pos *p, int depth, uint64_t *nodes) {
-  return search_common(p, depth, NULL, 0, nodes, NULL, 0, 1);
+  return search_common(p, depth, NULL, 0, nodes, NULL, 0, 1, 0);
 }
 
 /* ------------------------------------------------------------------ batches */
@@ -820,7 +854,8 @@ static void *batch_worker(void *arg) {
     if (j->kind == 0) j->counts[i] = rc_perft(&j->p[i], j->depth);
     else {
       uint64_t nd = 0;
-      j->scores[i] = rc_search(&j->p[i], j->depth, NULL, 0, &nd, NULL);
+      j->scores[i] = j->kind == 2 ? rc_search_id(&j->p[i], j->depth, NULL, 0, &nd, NULL)
+                                  : rc_search(&j->p[i], j->depth, NULL, 0, &nd, NULL);
       if (j->nodes) j->nodes[i] = nd;
     }
   }
@@ -850,6 +885,13 @@ void rc_search_batch(const rc_pos *p, int n, int depth, int32_t *scores, uint64_
   batch_job j;
   memset(&j, 0, sizeof(j));
   j.p = p; j.n = n; j.depth = depth; j.kind = 1; j.scores = scores; j.nodes = nodes;
+  run_batch(&j, threads);
+}
+
+void rc_search_id_batch(const rc_pos *p, int n, int depth, int32_t *scores, uint64_t *nodes, int threads) {
+  batch_job j;
+  memset(&j, 0, sizeof(j));
+  j.p = p; j.n = n; j.depth = depth; j.kind = 2; j.scores = scores; j.nodes = nodes;
   run_batch(&j, threads);
 }
 

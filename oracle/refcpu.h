@@ -100,13 +100,21 @@ int32_t  rc_eval(const uint64_t *board4);
  * *nodes receives moves made (src/zeta.cl:2283-2285). */
 int32_t  rc_search(const rc_pos *p, int depth, const uint64_t *hist, int ply_init,
                    uint64_t *nodes, uint32_t *bestmove);
+/* Iterative deepening 1..depth of rc_search (src/search.c:77-198), the
+ * killers and counter moves kept across the iterations: the score of the
+ * last iteration and the moves made over all of them -- the node count a
+ * single-worker device search without a transposition table must equal. */
+int32_t  rc_search_id(const rc_pos *p, int depth, const uint64_t *hist, int ply_init,
+                      uint64_t *nodes, uint32_t *bestmove);
 /* Same with plain minimax (no pruning), for small depths. */
 int32_t  rc_minimax(const rc_pos *p, int depth, uint64_t *nodes);
 /* Same as rc_search with the move ordering reversed (worst first). */
 int32_t  rc_search_reversed(const rc_pos *p, int depth, uint64_t *nodes);
-/* Multi-threaded batch search. */
+/* Multi-threaded batch search (rc_search / rc_search_id per position). */
 void     rc_search_batch(const rc_pos *p, int n, int depth, int32_t *scores,
                          uint64_t *nodes, int threads);
+void     rc_search_id_batch(const rc_pos *p, int n, int depth, int32_t *scores,
+                            uint64_t *nodes, int threads);
 
 /* ---- seeded workload generation ---- */
 uint64_t rc_splitmix64(uint64_t *state);

@@ -61,6 +61,10 @@ def lib():
         L.rc_search_reversed.restype = i32
         L.rc_search_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
         L.rc_search_batch.restype = None
+        L.rc_search_id_batch.argtypes = L.rc_search_batch.argtypes
+        L.rc_search_id_batch.restype = None
+        L.rc_search_id.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+        L.rc_search_id.restype = i32
         L.rc_gen_playouts.argtypes = [u64, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.rc_gen_playouts.restype = ctypes.c_int
         L.rc_init()
@@ -149,6 +153,17 @@ def search_batch(pos, depth, threads=None):
     return scores, nodes
 
 
+def search_id_batch(pos, depth, threads=None):
+    """Iterative deepening 1..depth per position: (scores, moves made over all
+    iterations) -- what a single-worker device search without a TT returns."""
+    pos = np.ascontiguousarray(pos, dtype=POS_DTYPE)
+    scores = np.zeros(len(pos), dtype=np.int32)
+    nodes = np.zeros(len(pos), dtype=np.uint64)
+    lib().rc_search_id_batch(_ptr(pos), len(pos), depth, _ptr(scores), _ptr(nodes),
+                             threads or os.cpu_count() or 1)
+    return scores, nodes
+
+
 def gen_playouts(seed, n, minply, maxply):
     out = np.zeros(n, dtype=POS_DTYPE)
     rc = lib().rc_gen_playouts(seed, n, minply, maxply, _ptr(out))
@@ -180,3 +195,12 @@ def move2can(move):
     if (pf >> 1) == 1 and (pt >> 1) != 1:
         s += {6: "q", 5: "r", 4: "b", 2: "n"}.get(pt >> 1, "")
     return s
+
+
+def move_from_can(pos, can):
+    """The legal move of pos written as can (e.g. "e2e4", "e7e8q")."""
+    for k in range(count_legal(pos)):
+        m = nth_legal(pos, k)
+        if move2can(m) == can or move2can(m) == can[:4]:
+            return m
+    raise ValueError("no legal move %s" % can)

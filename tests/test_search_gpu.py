@@ -56,6 +56,104 @@ def test_playout_positions_small_depth(zeta, oracle, depth):
     assert (got["score"] == want).all()
 
 
+@pytest.mark.parametrize("depth", [2, 3, 4, 5])
+def test_single_worker_tree_equals_oracle(zeta, oracle, depth):
+    """One worker, no transposition table: the device search orders moves as
+    the reference's picker does (MVV-LVA / piece-square deltas, the ply's
+    killer at 1400 and the counter move of the previous move at 1300, highest
+    first, the earliest generated among equals; src/zeta.cl:2059-2082) and
+    keeps killers and counter moves across the iterations 1..depth, so it
+    makes exactly the moves the oracle's iterative deepening makes, root by
+    root -- the count pins the plain ordering, not only the score."""
+    pos = oracle.gen_playouts(3, 128, 20, 60)
+    want, wnodes = oracle.search_id_batch(pos, depth)
+    got = zeta.search(pos, depth, workers=1, tt_log2=0)
+    bad = np.nonzero((got["score"] != want) | (got["nodes"] != wnodes))[0]
+    print("depth", depth, "nodes", int(got["nodes"].sum()), int(wnodes.sum()), "mismatches", len(bad), bad[:8],
+          flush=True)
+    assert (got["status"] == 0).all()
+    assert len(bad) == 0
+
+
+def test_node_budget_single_worker(zeta, oracle):
+    """max_nodes stops a root at its node budget (src/zeta.cl:1945-1947):
+    status ZETA_EBUDGET, and the score is that of the deepest iteration the
+    worker completed -- the oracle's score at that depth."""
+    pos = oracle.gen_playouts(3, 32, 20, 60)
+    for budget in (1, 300, 5000):
+        got = zeta.search(pos, 9, workers=1, tt_log2=0, max_nodes=budget)
+        print("budget", budget, list(got["depth"]), list(got["status"]), flush=True)
+        assert set(got["status"].tolist()) <= {0, zeta.ZETA_EBUDGET}
+        assert (got["status"] == zeta.ZETA_EBUDGET).sum() >= 30
+        # a worker polls every 64 nodes: it stops within one poll of the budget
+        assert (got["nodes"] <= budget + 64).all()
+        for i in np.nonzero(got["depth"] > 0)[0][:8]:
+            assert got["score"][i] == oracle.search(pos[i:i + 1], int(got["depth"][i]))[0], i
+
+
+def test_node_budget_many_workers_returns(zeta, oracle):
+    """With helpers and a TT the first worker over its budget gives the root
+    up and no worker rejoins it: the batch returns, every root stopped."""
+    pos = oracle.gen_playouts(3, 64, 20, 60)
+    for workers, tt in ((8, 20), (32, 0)):
+        got = zeta.search(pos, 10, workers=workers, tt_log2=tt, max_nodes=2000)
+        print("workers", workers, "tt", tt, list(got["status"]), int(got["nodes"].sum()), flush=True)
+        assert (got["status"] == zeta.ZETA_EBUDGET).all()
+        # each worker stops within a poll of its budget
+        assert (got["nodes"] <= workers * (2000 + 64)).all()
+    # an unbudgeted search afterwards is unaffected
+    want, _ = oracle.search_batch(pos[:16], 3)
+    got = zeta.search(pos[:16], 3, workers=4, tt_log2=20)
+    assert (got["status"] == 0).all() and (got["score"] == want).all()
+
+
+def test_abort_before_a_batch(zeta, oracle):
+    pos = oracle.gen_playouts(3, 16, 20, 60)
+    zeta.search_abort(True)
+    try:
+        got = zeta.search(pos, 5, workers=4, tt_log2=20)
+    finally:
+        zeta.search_abort(False)
+    assert (got["status"] == zeta.ZETA_ECANCELED).all() and (got["nodes"] == 0).all()
+    want, _ = oracle.search_batch(pos, 3)
+    got = zeta.search(pos, 3, workers=4, tt_log2=20)
+    assert (got["status"] == 0).all() and (got["score"] == want).all()
+
+
+def test_abort_during_a_batch(zeta, oracle):
+    """zeta_search_abort from another host thread stops a running batch at
+    the workers' next poll; the roots it completed keep exact scores."""
+    import threading
+    import time
+    pos = oracle.gen_playouts(7, 256, 20, 60)
+    out = {}
+
+    def run():
+        t = time.perf_counter()
+        out["res"] = zeta.search(pos, 7, workers=8, tt_log2=22)
+        out["s"] = time.perf_counter() - t
+
+    th = threading.Thread(target=run)
+    th.start()
+    time.sleep(1.0)
+    t_abort = time.perf_counter()
+    zeta.search_abort(True)
+    th.join(60)
+    stopped_after = time.perf_counter() - t_abort
+    zeta.search_abort(False)
+    assert not th.is_alive()
+    got = out["res"]
+    st = got["status"]
+    print("finished", int((st == 0).sum()), "cancelled", int((st == zeta.ZETA_ECANCELED).sum()),
+          "returned %.2f s after the abort" % stopped_after, flush=True)
+    assert set(st.tolist()) <= {0, zeta.ZETA_ECANCELED}
+    assert (st == zeta.ZETA_ECANCELED).sum() > 0
+    assert stopped_after < 5.0
+    done = np.nonzero(st == 0)[0][:4]
+    for i in done:
+        assert got["depth"][i] == 7
+
+
 def test_workers_and_tt_do_not_change_scores(zeta, oracle):
     pos = oracle.gen_playouts(3, 64, 20, 60)
     want, _ = oracle.search_batch(pos, 4)

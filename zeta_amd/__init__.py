@@ -8,7 +8,9 @@ no gfx950 device is present, the calls raise.
 Reference interface mirrored (shintakezou/Zeta 099m):
   perft()        -> perft() + cl_run_perft  (src/search.c:225-271, src/clrun.c:979)
   search()       -> rootsearch() + cl_run_alphabeta (src/search.c:35-223, src/clrun.c:675)
-  setboard()     -> setboard (src/io.c:335-532), via the product's own C parser
+Positions come in as the reference's quad-bitboard records (POS_DTYPE); the
+FEN parser with the reference's setboard semantics is the front end's
+(zeta_amd/host/zeta_engine.cpp).
 """
 import ctypes
 import os
@@ -75,6 +77,7 @@ SERVE_DONE_DTYPE = np.dtype([("ticket", "<u8"), ("score", "<i4"), ("bestmove", "
                              ("depth", "<i4"), ("status", "<i4")])
 ZETA_EBUSY = -6
 ZETA_ECANCELED = -7
+ZETA_EBUDGET = -8
 ZETA_PENDING = 1
 ZETA_REQ_LEARNED = 1
 
@@ -103,6 +106,7 @@ def lib():
         L.zeta_set_timing.argtypes = [i32]
         L.zeta_search_batch.argtypes = [vp, i32, ctypes.POINTER(SearchOpts), vp]
         L.zeta_search_counters.argtypes = [vp, i32]
+        L.zeta_search_abort.argtypes = [i32]
         L.zeta_mlp_init.argtypes = [u64]
         L.zeta_mlp_set.argtypes = [vp, ctypes.c_size_t]
         L.zeta_mlp_get.argtypes = [vp, ctypes.c_size_t]
@@ -129,7 +133,7 @@ def lib():
                   "zeta_perft_batch", "zeta_perft_batch_device", "zeta_playouts",
                   "zeta_stats_get", "zeta_stats_reset", "zeta_set_timing", "zeta_dev_alloc",
                   "zeta_dev_free", "zeta_memcpy_htod", "zeta_memcpy_dtoh", "zeta_memcpy_dtod", "zeta_synchronize",
-                  "zeta_search_batch", "zeta_search_counters", "zeta_mlp_init", "zeta_mlp_set",
+                  "zeta_search_batch", "zeta_search_counters", "zeta_search_abort", "zeta_mlp_init", "zeta_mlp_set",
                   "zeta_mlp_get", "zeta_mlp_grad", "zeta_mlp_train", "zeta_records_reserve",
                   "zeta_records_count", "zeta_records_clear", "zeta_records_get", "zeta_records_put",
                   "zeta_mlp_score", "zeta_serve_start", "zeta_serve_cost", "zeta_serve_submit",
@@ -231,6 +235,13 @@ def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_ml
                    pv.ctypes.data if pv_len > 0 else None, int(pv_len))
     _check(lib().zeta_search_batch(_ptr(pos), len(pos), ctypes.byref(o), _ptr(out)), "zeta_search_batch")
     return (out, pv) if pv_len > 0 else out
+
+
+def search_abort(on=True):
+    """zeta_search_abort: stop a search running on another thread (and every
+    search started while on); on=False re-arms."""
+    ensure_init()
+    _check(lib().zeta_search_abort(1 if on else 0), "zeta_search_abort")
 
 
 def search_counters():

@@ -44,6 +44,10 @@ struct Ctx {
   int cu_count = 0;
   bool timing = false;
   bool serving = false;  // a persistent serving grid owns the stream (zeta_serve_start .. stop)
+  // search abort word (zeta_search_abort): coherent pinned host memory that
+  // the search workers poll; the device address of the same word
+  uint32_t *abort_host = nullptr;
+  const uint32_t *abort_dev = nullptr;
   // timing
   struct Pending {
     int id;

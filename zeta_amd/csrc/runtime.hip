@@ -144,6 +144,24 @@ extern "C" int zeta_init(const zeta_config *cfg) {
   c.cu_count = prop.multiProcessorCount;
   e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
   if (e != hipSuccess) return fail(ZETA_ENODEV, "hipStreamCreate: %s", hipGetErrorString(e));
+  void *ab = nullptr;
+  e = hipHostMalloc(&ab, 64, hipHostMallocCoherent | hipHostMallocMapped);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+    return fail(ZETA_ENOMEM, "hipHostMalloc of the abort word: %s", hipGetErrorString(e));
+  }
+  memset(ab, 0, 64);
+  void *abd = nullptr;
+  e = hipHostGetDevicePointer(&abd, ab, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(ab);
+    (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+    return fail(ZETA_ENODEV, "hipHostGetDevicePointer of the abort word: %s", hipGetErrorString(e));
+  }
+  c.abort_host = (uint32_t *)ab;
+  c.abort_dev = (const uint32_t *)abd;
   c.cfg = d;
   c.device = d.device;
   memset(c.kstat, 0, sizeof(c.kstat));
@@ -167,6 +185,9 @@ extern "C" int zeta_shutdown(void) {
   mlp_release();
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
   c.event_pool.clear();
+  if (c.abort_host) (void)hipHostFree(c.abort_host);
+  c.abort_host = nullptr;
+  c.abort_dev = nullptr;
   (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
   c.up = false;

@@ -48,8 +48,12 @@ constexpr s32 MATESCORE = 30000;
 constexpr int MSTACK = 1024;  // move-stack slots per worker (LDS)
 constexpr s32 USED = -32768;  // 16-bit move-ordering scores
 constexpr s32 TTMOVE_SCORE = 32000;  // above every other score (src/zeta.cl:2101)
-constexpr s32 KILLER0_SCORE = 1400;  // src/zeta.cl:2082
-constexpr s32 KILLER1_SCORE = 1300;
+// the killer (one per ply) and the counter move (indexed by the previous
+// move's squares), quiet moves only: src/zeta.cl:2073-2082
+constexpr s32 KILLER_SCORE = 1400;
+constexpr s32 COUNTER_SCORE = 1300;
+constexpr int NCOUNTER = 64 * 64;  // counter-move table entries per worker (LDS)
+constexpr u32 DEFERRED = 0x80000000u;  // move-stack flag: deferred by ABDADA (move codes use bits 0-29)
 constexpr int ABDADA_MIN_DEPTH = 2;
 constexpr int POLL_EVERY = 64;
 
@@ -112,7 +116,7 @@ struct SearchArgs {
   u64 *nodes;
   s32 *depth_done;
   u32 *err;
-  const u32 *abort_flag;
+  const u32 *habort;  // abort word in coherent pinned host memory (zeta_search_abort)
   u64 max_nodes;  // per worker and root (0 = none)
   u64 *ctr;       // batch counters, see zeta_search_counters
   // learned ordering
@@ -303,6 +307,14 @@ __device__ __forceinline__ u32 tt_store(const SearchArgs &a, const Node &n, int 
   return 1;
 }
 
+// A counter move is a quiet move, which its from- and to-square and its
+// piece determine (sqcpt = to, no captured piece, pto = pfrom or the queen a
+// pawn on the last rank becomes): 16 bits hold it, and a quiet candidate
+// equals the stored move exactly when its 16 bits do (a capture never does,
+// as in the reference's full-code comparison, src/zeta.cl:2072)
+__device__ __forceinline__ u16 counter_code(u32 m) { return (u16)((m & 0xFFFu) | (mv_pfrom(m) << 12)); }
+__device__ __forceinline__ int counter_index(u32 prev) { return (int)(mv_from(prev) * 64 + mv_to(prev)); }
+
 __device__ __forceinline__ void busy_release(const SearchArgs &a, u64 hash, u64 cr, int lane) {
   const u32 bidx = (u32)(tt_key(hash, cr) >> 40) & a.busymask;
   if (lane == 0) atomicSub(&a.busy[bidx], 1u);
@@ -426,7 +438,8 @@ __device__ void mlp_score_moves(const SearchArgs &a, const Pos &p, int mbeg, int
 }
 
 // training pairs at a cut node whose cutoff move is quiet (no capture, no
-// promotion, not a killer) -- the moves the learned ordering ranks: the
+// promotion, neither the killer nor the counter move) -- the moves the
+// learned ordering ranks: the
 // cutoff move against every quiet move searched before it (scores marked
 // USED, the hard negatives) and against a hash-selected sample of about two
 // quiet moves never searched; wave-compacted
@@ -435,16 +448,17 @@ __device__ __forceinline__ bool plain_quiet(u32 m) {
 }
 
 __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u32 *mmove, const int16_t *mscore,
-                          u32 k0, u32 k1, u32 &nrec) {
-  if (!plain_quiet(n.move) || n.move == k0 || n.move == k1) return;
+                          u32 killer, u32 counter, u32 &nrec) {
+  // counter: the counter move's 16-bit code (counter_code)
+  if (!plain_quiet(n.move) || n.move == killer || counter_code(n.move) == counter) return;
   const u32 thresh = n.mcnt > 2 ? (u32)(2 * 256 / n.mcnt) : 256u;
   for (int c0 = 0; c0 < n.mcnt; c0 += 64) {
     const int k = c0 + lane;
     bool neg = false;
     u32 m = 0;
     if (k < n.mcnt) {
-      m = mmove[n.mbeg + k];
-      if (m != n.move && m != k0 && m != k1 && plain_quiet(m)) {
+      m = mmove[n.mbeg + k] & ~DEFERRED;
+      if (m != n.move && m != killer && counter_code(m) != counter && plain_quiet(m)) {
         if (mscore[n.mbeg + k] == (int16_t)USED) {
           neg = true;
         } else {
@@ -683,8 +697,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
   __shared__ Ply ply[MAXPLY + 2];
   __shared__ u32 mmove[MSTACK];
   __shared__ int16_t mscore[MSTACK];
-  __shared__ u8 mflag[MSTACK];
-  __shared__ u32 killers[MAXPLY + 2][2];
+  __shared__ __align__(16) u16 ctab[NCOUNTER];  // counter moves by the previous move's squares (cleared 16 B at a time)
+  __shared__ u32 killers[MAXPLY + 2];
   __shared__ float hpos[zm::NH1];
   __shared__ u8 mstart[64];  // move-list start marks (lane + 1 at its offset)
   const int lane = threadIdx.x;
@@ -702,7 +716,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       r = serve_pick(a, lane, ticket);
     } else {
       int rr = -1;
-      if (lane == 0) rr = *a.abort_flag ? -1 : pick_root(a, hstart);
+      if (lane == 0) rr = sys_load32(a.habort) ? -1 : pick_root(a, hstart);
       r = uf(rr);
     }
     if (r < 0) return;
@@ -724,7 +738,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       hl = uf(a.hlen[r]);
       hl = hl < 0 ? 0 : (hl > ZETA_HISTORY ? ZETA_HISTORY : hl);
     }
-    for (int k = lane; k < (MAXPLY + 2) * 2; k += 64) killers[k >> 1][k & 1] = 0;
+    // killers and counter moves start empty for every root (they persist
+    // across its iterations), so a root's search does not depend on which
+    // worker searched what before
+    for (int k = lane; k < MAXPLY + 2; k += 64) killers[k] = 0;
+    for (int k = lane; k < NCOUNTER / 8; k += 64) ((uint4 *)ctab)[k] = make_uint4(0u, 0u, 0u, 0u);
     wsync();
 
     // per-root statistics in one vector register (counter k in lane k) rather
@@ -793,8 +811,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               if (serving) {
                 d = serve_abort(a, r, ticket);
               } else {
-                d = __atomic_load_n(&a.done[r], __ATOMIC_RELAXED) | __atomic_load_n(a.abort_flag, __ATOMIC_RELAXED);
-                if (a.max_nodes && nodes > a.max_nodes) d = 1;
+                // stop when the root is finished or given up, or the host aborts
+                d = (__atomic_load_n(&a.done[r], __ATOMIC_RELAXED) != 0u || sys_load32(a.habort) != 0u) ? 1u : 0u;
+                if (!d && a.max_nodes && nodes > a.max_nodes) {
+                  // over the node budget: give the root up, so that no worker
+                  // picks it again (a fresh budget on the same iteration could
+                  // repeat forever); its result is this worker's deepest
+                  // completed iteration
+                  if (atomicCAS(&a.done[r], 0u, 3u) == 0u) {
+                    a.score[r] = final_score;
+                    a.bestmove[r] = final_move;
+                  }
+                  d = 1;
+                }
               }
             }
             if (uf(d)) {
@@ -863,19 +892,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             failed = true;
             break;
           }
+          // killer and counter move (quiescence nodes have no quiet moves)
+          const u32 kil = killers[sd], cm = n.qs ? 0u : (u32)ctab[counter_index(n.lastmove)];
           {
-            const u32 k0 = killers[sd][0], k1 = killers[sd][1];
             auto put = [&](int k, int from, int to) {
               const u32 m = make_code(n.p, from, to);
               s32 sc = order_score(n.p.stm, m);
               sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
               if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
-                if (m == k0) sc = KILLER0_SCORE;
-                else if (m == k1) sc = KILLER1_SCORE;
+                if (m == kil) sc = KILLER_SCORE;
+                else if (counter_code(m) == cm) sc = COUNTER_SCORE;
               }
               mmove[k] = m;
               mscore[k] = (int16_t)sc;
-              mflag[k] = 0;
             };
             if (!n.qs && total <= 64) {
               // lane j writes move j of the list (the same order as the loop
@@ -907,23 +936,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             wsync();
             mlp_score_moves(a, n.p, n.mbeg, total, lane, hpos, mmove, mscore);
             // the network orders the quiet moves (no capture, no promotion,
-            // not a killer) among themselves, in a band below every capture
-            // and killer, which keep the plain scores -- the plain ordering
-            // also ranks every capture (score >= 400) above every quiet move
-            const u32 k0 = killers[sd][0], k1 = killers[sd][1];
+            // neither killer nor counter move) among themselves, in a band
+            // below every capture, killer and counter move, which keep the
+            // plain scores -- the plain ordering also ranks every capture
+            // (score >= 400) above every quiet move
             for (int k = lane; k < total; k += 64) {
               const u32 m = mmove[n.mbeg + k];
               const bool quiet = ((mv_pcpt(m) >> 1) & 7) == PNONE && mv_pto(m) == mv_pfrom(m);
               s32 sc;
-              if (quiet && m != k0 && m != k1) {
+              if (quiet && m != kil && counter_code(m) != cm) {
                 sc = (s32)mscore[n.mbeg + k] / 2;
                 sc = -16000 + (sc > 15000 ? 15000 : (sc < -15000 ? -15000 : sc));
               } else {
                 sc = order_score(n.p.stm, m);
                 sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
                 if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
-                  if (m == k0) sc = KILLER0_SCORE;
-                  else if (m == k1) sc = KILLER1_SCORE;
+                  if (m == kil) sc = KILLER_SCORE;
+                  else if (counter_code(m) == cm) sc = COUNTER_SCORE;
                 }
               }
               mscore[n.mbeg + k] = (int16_t)sc;
@@ -1007,8 +1036,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           u32 bestkey = 0;
           for (int k = lane; k < n.mcnt; k += 64) {
             const s32 sc = mscore[n.mbeg + k];
-            const u8 fl = mflag[n.mbeg + k];
-            const bool ok = sc != USED && (n.iter == 1 ? fl == 0 : fl != 0);
+            const bool fl = (mmove[n.mbeg + k] & DEFERRED) != 0u;
+            const bool ok = sc != USED && (n.iter == 1 ? !fl : fl);
             if (ok) {
               const u32 key = ((u32)(sc + 32768) << 16) | (0xFFFFu - (u32)k);
               bestkey = key > bestkey ? key : bestkey;
@@ -1026,7 +1055,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             continue;
           }
           const int k = n.mbeg + (int)(0xFFFFu - (bestkey & 0xFFFFu));
-          const u32 m = uf(mmove[k]);
+          const u32 m = uf(mmove[k]) & ~DEFERRED;
           const s32 sk = (s32)(bestkey >> 16) - 32768;
           wsync();
           if (lane == 0) mscore[k] = (int16_t)USED;
@@ -1052,7 +1081,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             if (n.iter == 1 && b && (!serving || n.searched > 0)) {
               if (lane == 0) {
                 mscore[k] = (int16_t)sk;
-                mflag[k] = 1;
+                mmove[k] = m | DEFERRED;
               }
               n.ndefer++;
               lcount(cvec, C_DEFER, 1);
@@ -1116,12 +1145,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 1 && n.mcnt >= 2 &&
               ((u32)(n.hash >> 40) & a.rec_mask) == 0) {
             u32 nrec = 0;
-            log_pairs(a, n, lane, mmove, mscore, killers[sd][0], killers[sd][1], nrec);
+            log_pairs(a, n, lane, mmove, mscore, killers[sd], (u32)ctab[counter_index(n.lastmove)], nrec);
             lcount(cvec, C_RECORDS, nrec);
           }
-          if (((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0 && killers[sd][0] != n.move) {
-            killers[sd][1] = killers[sd][0];
-            killers[sd][0] = n.move;
+          // a quiet move that fails high becomes the ply's killer and the
+          // counter move of the move before it (src/zeta.cl:1916-1933)
+          if (!n.qs && ((mv_pcpt(n.move) >> 1) & 7) == PNONE && lane == 0) {
+            killers[sd] = n.move;
+            ctab[counter_index(n.lastmove)] = counter_code(n.move);
           }
           v = n.alpha;
           lcount(cvec, C_TT_STORE, tt_store(a, n, sd, lane));
@@ -1237,8 +1268,7 @@ __global__ void k_clear_u64(u64 *p, size_t n) {
 }
 
 struct SearchState {
-  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, abortf, ctr, hist, hlen,
-      pv;
+  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, ctr, hist, hlen, pv;
   u64 tt_entries = 0;
   u64 last_ctr[C_NCTR] = {0};
 };
@@ -1254,7 +1284,7 @@ void *records_buffer(u64 *cap, u64 **count);
 void search_release() {
   zr::release(g_ss.tt); zr::release(g_ss.busy); zr::release(g_ss.roots); zr::release(g_ss.queue);
   zr::release(g_ss.done); zr::release(g_ss.nworkers); zr::release(g_ss.score); zr::release(g_ss.bestmove);
-  zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err); zr::release(g_ss.abortf);
+  zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err);
   zr::release(g_ss.ctr); zr::release(g_ss.hist); zr::release(g_ss.hlen); zr::release(g_ss.pv);
   g_ss.tt_entries = 0;
 }
@@ -1275,7 +1305,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
       (r = zr::ensure(g_ss.done, (size_t)n * 4)) || (r = zr::ensure(g_ss.nworkers, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.score, (size_t)n * 4)) || (r = zr::ensure(g_ss.bestmove, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.nodes, (size_t)n * 8)) || (r = zr::ensure(g_ss.depth_done, (size_t)n * 4)) ||
-      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.abortf, 64)) || (r = zr::ensure(g_ss.ctr, C_NCTR * 8)) ||
+      (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.ctr, C_NCTR * 8)) ||
       (r = zr::ensure(g_ss.busy, (size_t)busy_entries * 4)))
     return r;
   if (tt_entries) {
@@ -1294,7 +1324,6 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   ZCHECK(hipMemsetAsync(g_ss.nodes.p, 0, (size_t)n * 8, st));
   ZCHECK(hipMemsetAsync(g_ss.depth_done.p, 0, (size_t)n * 4, st));
   ZCHECK(hipMemsetAsync(g_ss.err.p, 0, 64, st));
-  ZCHECK(hipMemsetAsync(g_ss.abortf.p, 0, 64, st));
   ZCHECK(hipMemsetAsync(g_ss.ctr.p, 0, C_NCTR * 8, st));
   ZCHECK(hipMemsetAsync(g_ss.busy.p, 0, (size_t)busy_entries * 4, st));
 
@@ -1322,7 +1351,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.nodes = (u64 *)g_ss.nodes.p;
   sa.depth_done = (s32 *)g_ss.depth_done.p;
   sa.err = (u32 *)g_ss.err.p;
-  sa.abort_flag = (const u32 *)g_ss.abortf.p;
+  sa.habort = zr::ctx().abort_dev;
   sa.max_nodes = opts->max_nodes;
   sa.ctr = (u64 *)g_ss.ctr.p;
   sa.use_mlp = 0;
@@ -1384,12 +1413,19 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   ZCHECK(hipMemcpyAsync(g_ss.last_ctr, g_ss.ctr.p, sizeof(g_ss.last_ctr), hipMemcpyDeviceToHost, st));
   ZCHECK(hipStreamSynchronize(st));
   zr::time_collect();
+  // a root the workers left without a result was stopped by zeta_search_abort
+  const bool host_abort = __atomic_load_n(zr::ctx().abort_host, __ATOMIC_ACQUIRE) != 0u;
+  int incomplete = -1;
   for (int i = 0; i < n; i++) {
     res[i].score = score[i];
     res[i].bestmove = bm[i];
     res[i].nodes = nodes[i];
     res[i].depth = dd[i];
-    res[i].status = done[i] == 1 ? 0 : (done[i] == 2 ? ZETA_ENOMEM : ZETA_ESTATE);
+    res[i].status = done[i] == 1 ? ZETA_OK
+                  : done[i] == 2 ? ZETA_ENOMEM
+                  : done[i] == 3 ? ZETA_EBUDGET
+                  : host_abort ? ZETA_ECANCELED : ZETA_ESTATE;
+    if (res[i].status == ZETA_ESTATE && incomplete < 0) incomplete = i;
   }
   if (opts->pv && opts->pv_len > 0) {
     const int len = opts->pv_len > MAXPLY ? MAXPLY : opts->pv_len;
@@ -1404,8 +1440,16 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
       for (int k = 0; k < opts->pv_len; k++) opts->pv[(size_t)i * opts->pv_len + k] = k < len ? pvh[(size_t)i * len + k] : 0u;
   }
   if (err & 1u) return zr::fail(ZETA_ENOMEM, "search: move stack overflow (more than %d moves on one path)", MSTACK);
-  for (int i = 0; i < n; i++)
-    if (done[i] != 1 && opts->max_nodes == 0) return zr::fail(ZETA_ESTATE, "search: root %d did not complete", i);
+  if (incomplete >= 0) return zr::fail(ZETA_ESTATE, "search: root %d did not complete", incomplete);
+  return ZETA_OK;
+}
+
+extern "C" int zeta_search_abort(int on) {
+  // may run on another host thread while zeta_search_batch waits: it touches
+  // the abort word only
+  int r = zr::require_init();
+  if (r) return r;
+  __atomic_store_n(zr::ctx().abort_host, on ? 1u : 0u, __ATOMIC_RELEASE);
   return ZETA_OK;
 }
 
@@ -1641,6 +1685,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   sa.depth_done = (s32 *)S.depth_done.p;
   sa.err = (u32 *)S.err.p;
   sa.ctr = (u64 *)S.ctr.p;
+  sa.habort = zr::ctx().abort_dev;
   sa.use_mlp = S.use_mlp ? 1 : 0;
   sa.mlp_min_depth = o->mlp_min_depth;
   if (w) sa.w = *w;

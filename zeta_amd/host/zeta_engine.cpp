@@ -536,7 +536,10 @@ bool selftest() {
           ABNODECOUNT, KATS[done].nodes, KATS[done].depth, el, (uint64_t)(ABNODECOUNT / (el > 0 ? el : 1e-9)));
     }
   }
-  out("#> %" PRIu64 " of %" PRIu64 " tests passed\n", passed, todo);
+  // the reference's summary banner (src/test.c:144-146)
+  out("#\n###############################\n");
+  out("### passed %" PRIu64 " from %" PRIu64 " tests ###\n", passed, todo);
+  out("###############################\n");
   return passed == todo;
 }
 
