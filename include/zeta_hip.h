@@ -74,6 +74,17 @@ int zeta_device_info(char *buf, size_t n);
 #define ZETA_PERFT_NOBULK 1 /* visit every leaf (generate its moves) instead of counting at depth-1 */
 int zeta_perft_batch(const zeta_pos *pos, int n, int depth, uint64_t *counts);
 int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts, int flags);
+/* zeta_perft_batch_device without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * frontier lives in a fixed 256 MiB array the library holds on every device
+ * it runs on until the process ends; a level too large for it is walked in
+ * chunks.  n <= ZETA_NOALLOC_MAX_ROOTS, depth <= ZETA_NOALLOC_MAX_DEPTH, else
+ * ZETA_EINVAL.  Every call on a device shares that array: calls that overlap
+ * on one device race.  Same counts as zeta_perft_batch_device. */
+#define ZETA_NOALLOC_MAX_ROOTS 65536
+#define ZETA_NOALLOC_MAX_DEPTH 8
+int zeta_perft_batch_device_noalloc(const void *d_pos, int n, int depth, void *d_counts, int flags);
 
 /* Seeded random-playout positions generated on the GPU: position i tries
  * attempts a = 0,1,..; rng = splitmix64 seeded with

@@ -350,14 +350,46 @@ struct LevelBufs {
 struct PerftState {
   LevelBufs lev[MAXLEV];
   zr::DevBuf total, roots, counts, fails, cuts;
-  u64 *h_total = nullptr;          // pinned host copy of a level's pair total
-  hipEvent_t ev_total = nullptr;   // recorded after that copy
 };
 PerftState g_ps;
 
+// ---- the allocation-free entry point's scratch array: the frontier of a
+// call to zeta_perft_batch_device_noalloc lives here instead of in buffers
+// the library allocates.  One array per device (the whole code object is
+// loaded per device), held until the process ends; every call on a device
+// reuses it, so calls that overlap on one device race.
+constexpr size_t SCRATCH_BYTES = 256ULL << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+unsigned char *g_scratch_dev[SCRATCH_MAX_DEVICES];  // hipGetSymbolAddress of scratch, per device
+hipError_t g_scratch_err = hipSuccess;
+constexpr int NCUT_FIXED = 511;  // chunk cuts a level can take in the scratch array
+
+// The buffers one perft call works in: the library's growable ones, or views
+// into the scratch array with fixed capacities (never grown, never freed).
+struct PerftBufs {
+  LevelBufs *lev;
+  zr::DevBuf *total, *cuts;
+  u64 budget;  // most pairs a chunk may hold
+  bool fixed;
+};
+PerftBufs g_pb;
+LevelBufs g_fixed_lev[MAXLEV];
+zr::DevBuf g_fixed_total, g_fixed_cuts;
+
+void use_growable_bufs() {
+  g_pb.lev = g_ps.lev;
+  g_pb.total = &g_ps.total;
+  g_pb.cuts = &g_ps.cuts;
+  g_pb.budget = zr::ctx().cfg.frontier_budget;
+  g_pb.fixed = false;
+}
+
 int ensure_level(int L, size_t n) {
-  LevelBufs &b = g_ps.lev[L];
+  LevelBufs &b = g_pb.lev[L];
   if (b.cap >= n && b.cap) return ZETA_OK;
+  if (g_pb.fixed)
+    return zr::fail(ZETA_ENOMEM, "perft scratch: level %d needs %zu positions, holds %zu", L, n, b.cap);
   size_t cap = n + n / 4 + 1024;
   int r;
   if ((r = zr::ensure(b.b0, cap * 8)) || (r = zr::ensure(b.b1, cap * 8)) || (r = zr::ensure(b.b2, cap * 8)) ||
@@ -370,8 +402,10 @@ int ensure_level(int L, size_t n) {
 }
 
 int ensure_pairs(int L, size_t m) {
-  LevelBufs &b = g_ps.lev[L];
+  LevelBufs &b = g_pb.lev[L];
   if (b.paircap >= m && b.paircap) return ZETA_OK;
+  if (g_pb.fixed)
+    return zr::fail(ZETA_ENOMEM, "perft scratch: level %d needs %zu pairs, holds %zu", L, m, b.paircap);
   size_t cap = m + m / 8 + 1024;
   int r;
   if ((r = zr::ensure(b.pm, cap * 8))) return r;
@@ -380,7 +414,7 @@ int ensure_pairs(int L, size_t m) {
 }
 
 Front front_of(int L) {
-  LevelBufs &b = g_ps.lev[L];
+  LevelBufs &b = g_pb.lev[L];
   return Front{(u64 *)b.b0.p, (u64 *)b.b1.p, (u64 *)b.b2.p, (u64 *)b.b3.p,
                (u64 *)b.cr.p, (u32 *)b.meta.p, (u32 *)b.cnt.p};
 }
@@ -389,7 +423,7 @@ inline unsigned blocks(size_t n, int per) { return (unsigned)((n + per - 1) / pe
 
 // exclusive scan of level L's counts [i0, i1) into lev[L].off[0..n]; returns total
 int scan_counts(int L, size_t i0, size_t i1, u64 *total) {
-  LevelBufs &b = g_ps.lev[L];
+  LevelBufs &b = g_pb.lev[L];
   const size_t n = i1 - i0;
   const size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   const u32 *cnt = (const u32 *)b.cnt.p + i0;
@@ -398,22 +432,19 @@ int scan_counts(int L, size_t i0, size_t i1, u64 *total) {
     return ZETA_OK;
   }
   ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_reduce, (unsigned)nb, TPB, 0, cnt, n, (u64 *)b.bsum.p);
-  ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_top, 1, TPB, 0, (u64 *)b.bsum.p, nb, (u64 *)g_ps.total.p);
+  ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_top, 1, TPB, 0, (u64 *)b.bsum.p, nb, (u64 *)g_pb.total->p);
   ZLAUNCH(zr::K_PERFT_SCAN, n, k_scan_down, (unsigned)nb, TPB, 0, cnt, n, (const u64 *)b.bsum.p, (u64 *)b.off.p);
-  if (!g_ps.h_total) {
-    ZCHECK(hipHostMalloc((void **)&g_ps.h_total, 8, hipHostMallocDefault));
-    ZCHECK(hipEventCreateWithFlags(&g_ps.ev_total, hipEventDisableTiming));
-  }
-  ZCHECK(hipMemcpyAsync(g_ps.h_total, g_ps.total.p, 8, hipMemcpyDeviceToHost, zr::ctx().stream));
-  ZCHECK(hipEventRecord(g_ps.ev_total, zr::ctx().stream));
+  // into the pinned word zeta_init set aside for it
+  ZCHECK(hipMemcpyAsync(zr::ctx().perft_total_host, g_pb.total->p, 8, hipMemcpyDeviceToHost, zr::ctx().stream));
+  ZCHECK(hipEventRecord(zr::ctx().ev_total, zr::ctx().stream));
   *total = ~0ULL;  // known after wait_total()
   return ZETA_OK;
 }
 
 // the total copied by scan_counts (the stream may run on: only the copy is waited for)
 int wait_total(u64 *total) {
-  ZCHECK(hipEventSynchronize(g_ps.ev_total));
-  *total = *g_ps.h_total;
+  ZCHECK(hipEventSynchronize(zr::ctx().ev_total));
+  *total = *zr::ctx().perft_total_host;
   return ZETA_OK;
 }
 
@@ -422,15 +453,15 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
   int r = scan_counts(L, i0, i1, &total);
   if (r) return r;
   if (i1 == i0) return ZETA_OK;
-  const u64 budget = zr::ctx().cfg.frontier_budget;
+  const u64 budget = g_pb.budget;
+  LevelBufs &lv = g_pb.lev[L];
   // the expansion is queued before the total is known and writes into the
   // current pair buffer when the total fits it (k_expand checks), so the
   // host's wait for the total overlaps the expansion instead of preceding it
-  const u64 spec_cap = g_ps.lev[L].paircap < budget ? g_ps.lev[L].paircap : budget;
+  const u64 spec_cap = lv.paircap < budget ? lv.paircap : budget;
   if (spec_cap)
     ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, XTPB), XTPB, 0, front_of(L), i0, i1,
-            (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p, (const u64 *)g_ps.total.p,
-            spec_cap);
+            (const u64 *)lv.off.p - i0, (u64)0, (u64 *)lv.pm.p, (const u64 *)g_pb.total->p, spec_cap);
   if ((r = wait_total(&total))) return r;
   if (total == 0) return ZETA_OK;
   // a level with more pairs than the budget runs in chunks of parents, each
@@ -441,14 +472,27 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
     const u64 bp = budget - 256;
     const u64 nb = total / bp + 1;
     const size_t n = i1 - i0;
-    if ((r = zr::ensure(g_ps.cuts, (size_t)(nb + 1) * 16))) return r;
-    u64 *d_idx = (u64 *)g_ps.cuts.p, *d_off = d_idx + nb + 1;
-    ZLAUNCH(zr::K_PERFT_SCAN, n, k_cuts, blocks(n, TPB), TPB, 0, (const u64 *)g_ps.lev[L].off.p, n, bp, nb, d_idx,
-            d_off);
-    std::vector<u64> cut(2 * (nb + 1));
-    ZCHECK(hipMemcpyAsync(cut.data(), d_idx, cut.size() * 8, hipMemcpyDeviceToHost, zr::ctx().stream));
+    if (g_pb.fixed) {
+      if (nb > (u64)NCUT_FIXED)
+        return zr::fail(ZETA_ENOMEM, "perft scratch: level %d needs %llu chunks, holds %d", L,
+                        (unsigned long long)nb, NCUT_FIXED);
+    } else if ((r = zr::ensure(g_ps.cuts, (size_t)(nb + 1) * 16))) {
+      return r;
+    }
+    u64 *d_idx = (u64 *)g_pb.cuts->p, *d_off = d_idx + nb + 1;
+    ZLAUNCH(zr::K_PERFT_SCAN, n, k_cuts, blocks(n, TPB), TPB, 0, (const u64 *)lv.off.p, n, bp, nb, d_idx, d_off);
+    // the cuts come back into a vector, or into a fixed array on the stack
+    // when the call may not allocate
+    std::vector<u64> cutv;
+    u64 cutf[2 * (NCUT_FIXED + 1)];
+    u64 *cut = cutf;
+    if (!g_pb.fixed) {
+      cutv.resize(2 * (nb + 1));
+      cut = cutv.data();
+    }
+    ZCHECK(hipMemcpyAsync(cut, d_idx, 2 * (nb + 1) * 8, hipMemcpyDeviceToHost, zr::ctx().stream));
     ZCHECK(hipStreamSynchronize(zr::ctx().stream));
-    const u64 *cidx = cut.data(), *coff = cut.data() + nb + 1;
+    const u64 *cidx = cut, *coff = cut + nb + 1;
     for (u64 c = 0; c < nb; c++) {
       const size_t a0 = (size_t)cidx[c], a1 = (size_t)cidx[c + 1];
       if (a1 <= a0) continue;
@@ -456,14 +500,14 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
       if (m == 0) continue;
       if ((r = ensure_pairs(L, m))) return r;
       ZLAUNCH(zr::K_PERFT_EXPAND, a1 - a0, k_expand, blocks(a1 - a0, XTPB), XTPB, 0, front_of(L), i0 + a0, i0 + a1,
-              (const u64 *)g_ps.lev[L].off.p - i0, coff[c], (u64 *)g_ps.lev[L].pm.p, (const u64 *)nullptr, (u64)0);
+              (const u64 *)lv.off.p - i0, coff[c], (u64 *)lv.pm.p, (const u64 *)nullptr, (u64)0);
       if (L + 1 == D - 1) {
-        ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u64 *)g_ps.lev[L].pm.p,
+        ZLAUNCH(zr::K_PERFT_LEAF, m, k_leaf, blocks(m, TPB), TPB, 0, front_of(L), (const u64 *)lv.pm.p,
                 (size_t)m, d_counts, nobulk);
       } else {
         if ((r = ensure_level(L + 1, m))) return r;
         ZLAUNCH(zr::K_PERFT_APPLY, m, k_apply, blocks(m, TPB), TPB, 0, front_of(L),
-                (const u64 *)g_ps.lev[L].pm.p, (size_t)m, front_of(L + 1));
+                (const u64 *)lv.pm.p, (size_t)m, front_of(L + 1));
         if ((r = process(L + 1, 0, m, D, d_counts, nobulk))) return r;
       }
     }
@@ -472,16 +516,16 @@ int process(int L, size_t i0, size_t i1, int D, u64 *d_counts, int nobulk) {
   if (!spec_cap || total > spec_cap) {
     if ((r = ensure_pairs(L, total))) return r;
     ZLAUNCH(zr::K_PERFT_EXPAND, i1 - i0, k_expand, blocks(i1 - i0, XTPB), XTPB, 0, front_of(L), i0, i1,
-            (const u64 *)g_ps.lev[L].off.p - i0, (u64)0, (u64 *)g_ps.lev[L].pm.p, (const u64 *)nullptr, (u64)0);
+            (const u64 *)lv.off.p - i0, (u64)0, (u64 *)lv.pm.p, (const u64 *)nullptr, (u64)0);
   }
   if (L + 1 == D - 1) {
-    ZLAUNCH(zr::K_PERFT_LEAF, total, k_leaf, blocks(total, TPB), TPB, 0, front_of(L),
-            (const u64 *)g_ps.lev[L].pm.p, (size_t)total, d_counts, nobulk);
+    ZLAUNCH(zr::K_PERFT_LEAF, total, k_leaf, blocks(total, TPB), TPB, 0, front_of(L), (const u64 *)lv.pm.p,
+            (size_t)total, d_counts, nobulk);
     return ZETA_OK;
   }
   if ((r = ensure_level(L + 1, total))) return r;
-  ZLAUNCH(zr::K_PERFT_APPLY, total, k_apply, blocks(total, TPB), TPB, 0, front_of(L),
-          (const u64 *)g_ps.lev[L].pm.p, (size_t)total, front_of(L + 1));
+  ZLAUNCH(zr::K_PERFT_APPLY, total, k_apply, blocks(total, TPB), TPB, 0, front_of(L), (const u64 *)lv.pm.p,
+          (size_t)total, front_of(L + 1));
   return process(L + 1, 0, total, D, d_counts, nobulk);
 }
 
@@ -492,7 +536,7 @@ int perft_device(const void *d_pos, int n, int depth, u64 *d_counts, int flags) 
   const int D = nobulk ? depth + 1 : depth;
   if (D >= MAXLEV) return zr::fail(ZETA_EINVAL, "perft depth %d exceeds %d", depth, MAXLEV - 2);
   int r;
-  if ((r = zr::ensure(g_ps.total, 64))) return r;
+  if (!g_pb.fixed && (r = zr::ensure(g_ps.total, 64))) return r;
   if (depth == 0) {
     ZLAUNCH(zr::K_PERFT_ROOTS, n, k_fill_u64, blocks(n, TPB), TPB, 0, d_counts, n, (u64)1);
     return ZETA_OK;
@@ -500,7 +544,7 @@ int perft_device(const void *d_pos, int n, int depth, u64 *d_counts, int flags) 
   if ((r = ensure_level(0, (size_t)n))) return r;
   ZLAUNCH(zr::K_PERFT_ROOTS, n, k_roots, blocks(n, TPB), TPB, 0, (const u64 *)d_pos, n, front_of(0));
   if (D == 1) {
-    ZLAUNCH(zr::K_PERFT_ROOTS, n, k_u32_to_u64, blocks(n, TPB), TPB, 0, (const u32 *)g_ps.lev[0].cnt.p, d_counts,
+    ZLAUNCH(zr::K_PERFT_ROOTS, n, k_u32_to_u64, blocks(n, TPB), TPB, 0, (const u32 *)g_pb.lev[0].cnt.p, d_counts,
             n);
     return ZETA_OK;
   }
@@ -522,21 +566,122 @@ void perft_release() {
   zr::release(g_ps.counts);
   zr::release(g_ps.fails);
   zr::release(g_ps.cuts);
-  if (g_ps.h_total) (void)hipHostFree(g_ps.h_total);
-  g_ps.h_total = nullptr;
-  if (g_ps.ev_total) (void)hipEventDestroy(g_ps.ev_total);
-  g_ps.ev_total = nullptr;
 }
 
 extern "C" int zeta_perft_batch_device(const void *d_pos, int n, int depth, void *d_counts, int flags) {
   int r = zr::require_up();
   if (r) return r;
   if ((!d_pos || !d_counts) && n > 0) return zr::fail(ZETA_EINVAL, "perft: null pointer");
+  use_growable_bufs();
   r = perft_device(d_pos, n, depth, (u64 *)d_counts, flags);
   hipError_t e = hipStreamSynchronize(zr::ctx().stream);
   zr::time_collect();
   if (r) return r;
   if (e != hipSuccess) return zr::hip_fail(e, "perft sync", __FILE__, __LINE__);
+  return ZETA_OK;
+}
+
+// zeta_init, with the device current: where this device holds the scratch
+// array (the array's name in host code is only a stand-in for it)
+int perft_setup() {
+  const int dev = zr::ctx().device;
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return zr::fail(ZETA_ENODEV, "perft: device %d beyond %d", dev,
+                                                             SCRATCH_MAX_DEVICES);
+  if (g_scratch_dev[dev]) return ZETA_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    // the other entry points do not need the array: the allocation-free one
+    // reports this error when it is called
+    (void)hipGetLastError();
+    g_scratch_err = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    return ZETA_OK;
+  }
+  g_scratch_dev[dev] = (unsigned char *)ptr;
+  return ZETA_OK;
+}
+
+// Views of the scratch array for a call of D - 1 frontier levels: every level
+// holds up to C positions (5 x u64 planes, meta, count, offset, scan sums)
+// and C pairs; C is the largest that fits, and it is the chunk budget, so no
+// level and no chunk ever needs more.  Returns C, 0 when it is below what
+// the roots need.
+size_t fixed_layout(unsigned char *base, int D, size_t n) {
+  const int nl = D - 1 > 1 ? D - 1 : 1;
+  const size_t align = 256, head = 2 * align + 2 * (NCUT_FIXED + 1) * 8;
+  // per level: 9 arrays, each rounded up to the alignment
+  const size_t slack = 9 * align + 8 * 3;
+  const size_t per_pos = 5 * 8 + 4 + 4 + 8 + 8;  // planes, meta, cnt, off, pairs
+  size_t C = (SCRATCH_BYTES - head) / nl;
+  C = C > slack ? (C - slack) / per_pos : 0;
+  C -= C / (SCAN_TILE / 8) + 2;  // room for the scan sums (8 B per SCAN_TILE positions)
+  if (C < 1024 || C < n) return 0;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    void *p = base + o;
+    o += (bytes + align - 1) / align * align;
+    return p;
+  };
+  g_fixed_total.p = take(64);
+  g_fixed_total.bytes = 64;
+  g_fixed_cuts.p = take(2 * (NCUT_FIXED + 1) * 8);
+  g_fixed_cuts.bytes = 2 * (NCUT_FIXED + 1) * 8;
+  for (int L = 0; L < MAXLEV; L++) {
+    LevelBufs &b = g_fixed_lev[L];
+    b = LevelBufs();
+    if (L >= nl) continue;
+    zr::DevBuf *planes[5] = {&b.b0, &b.b1, &b.b2, &b.b3, &b.cr};
+    for (auto *q : planes) {
+      q->p = take(C * 8);
+      q->bytes = C * 8;
+    }
+    b.meta.p = take(C * 4);
+    b.meta.bytes = C * 4;
+    b.cnt.p = take(C * 4);
+    b.cnt.bytes = C * 4;
+    b.off.p = take((C + 1) * 8);
+    b.off.bytes = (C + 1) * 8;
+    b.bsum.p = take((C / SCAN_TILE + 2) * 8);
+    b.bsum.bytes = (C / SCAN_TILE + 2) * 8;
+    b.pm.p = take(C * 8);
+    b.pm.bytes = C * 8;
+    b.cap = C;
+    b.paircap = C;
+  }
+  if (o > SCRATCH_BYTES) return 0;
+  return C;
+}
+
+extern "C" int zeta_perft_batch_device_noalloc(const void *d_pos, int n, int depth, void *d_counts, int flags) {
+  int r = zr::require_up();
+  if (r) return r;
+  if ((!d_pos || !d_counts) && n > 0) return zr::fail(ZETA_EINVAL, "perft (no-alloc): null pointer");
+  if (n < 0 || n > ZETA_NOALLOC_MAX_ROOTS)
+    return zr::fail(ZETA_EINVAL, "perft (no-alloc): %d roots, at most %d", n, ZETA_NOALLOC_MAX_ROOTS);
+  if (depth < 0 || depth > ZETA_NOALLOC_MAX_DEPTH)
+    return zr::fail(ZETA_EINVAL, "perft (no-alloc): depth %d, at most %d", depth, ZETA_NOALLOC_MAX_DEPTH);
+  const int dev = zr::ctx().device;
+  unsigned char *base = (dev >= 0 && dev < SCRATCH_MAX_DEVICES) ? g_scratch_dev[dev] : nullptr;
+  if (!base)
+    return zr::fail(ZETA_ENODEV, "perft (no-alloc): no scratch array on device %d (hipGetSymbolAddress: %s)", dev,
+                    hipGetErrorString(g_scratch_err));
+  const int D = (flags & ZETA_PERFT_NOBULK) ? depth + 1 : depth;
+  const size_t C = fixed_layout(base, D, (size_t)n);
+  if (!C) return zr::fail(ZETA_EINVAL, "perft (no-alloc): %d roots at depth %d do not fit the scratch array", n, depth);
+  g_pb.lev = g_fixed_lev;
+  g_pb.total = &g_fixed_total;
+  g_pb.cuts = &g_fixed_cuts;
+  g_pb.budget = C;
+  g_pb.fixed = true;
+  // no kernel timing here: its events come from a pool that may grow
+  const bool timing = zr::ctx().timing;
+  zr::ctx().timing = false;
+  r = perft_device(d_pos, n, depth, (u64 *)d_counts, flags);
+  zr::ctx().timing = timing;
+  use_growable_bufs();
+  const hipError_t e = hipStreamSynchronize(zr::ctx().stream);
+  if (r) return r;
+  if (e != hipSuccess) return zr::hip_fail(e, "perft (no-alloc) sync", __FILE__, __LINE__);
   return ZETA_OK;
 }
 
@@ -548,6 +693,7 @@ extern "C" int zeta_perft_batch(const zeta_pos *pos, int n, int depth, uint64_t 
   if ((r = zr::ensure(g_ps.roots, (size_t)n * sizeof(zeta_pos))) || (r = zr::ensure(g_ps.counts, (size_t)n * 8)))
     return r;
   ZCHECK(hipMemcpyAsync(g_ps.roots.p, pos, (size_t)n * sizeof(zeta_pos), hipMemcpyHostToDevice, zr::ctx().stream));
+  use_growable_bufs();
   r = perft_device(g_ps.roots.p, n, depth, (u64 *)g_ps.counts.p, 0);
   if (r) {
     (void)hipStreamSynchronize(zr::ctx().stream);

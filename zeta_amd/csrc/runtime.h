@@ -44,10 +44,14 @@ struct Ctx {
   int cu_count = 0;
   bool timing = false;
   bool serving = false;  // a persistent serving grid owns the stream (zeta_serve_start .. stop)
-  // search abort word (zeta_search_abort): coherent pinned host memory that
-  // the search workers poll; the device address of the same word
+  // 64 bytes of coherent pinned host memory, allocated by zeta_init:
+  // bytes 0-3 the search abort word (zeta_search_abort) that the search
+  // workers poll at abort_dev, bytes 8-15 the perft level total the host
+  // waits for (perft_total_host, after ev_total)
   uint32_t *abort_host = nullptr;
   const uint32_t *abort_dev = nullptr;
+  uint64_t *perft_total_host = nullptr;
+  hipEvent_t ev_total = nullptr;
   // timing
   struct Pending {
     int id;

@@ -120,9 +120,14 @@ void time_collect() {
 
 using namespace zr;
 
+// per-device set-up of the perft subsystem (perft.hip): the address of the
+// allocation-free entry point's scratch array on the current device
+int perft_setup();
+
 extern "C" int zeta_init(const zeta_config *cfg) {
   Ctx &c = ctx();
   if (c.up) return ZETA_OK;
+  int rc = ZETA_OK;
   zeta_config d{};
   if (cfg) d = *cfg;
   if (d.frontier_budget == 0) d.frontier_budget = 1ULL << 28;
@@ -162,6 +167,20 @@ extern "C" int zeta_init(const zeta_config *cfg) {
   }
   c.abort_host = (uint32_t *)ab;
   c.abort_dev = (const uint32_t *)abd;
+  c.perft_total_host = (uint64_t *)((char *)ab + 8);
+  e = hipEventCreateWithFlags(&c.ev_total, hipEventDisableTiming);
+  if (e == hipSuccess && (rc = perft_setup()) != ZETA_OK) e = hipErrorUnknown;
+  if (e != hipSuccess) {
+    if (c.ev_total) (void)hipEventDestroy(c.ev_total);
+    c.ev_total = nullptr;
+    (void)hipHostFree(ab);
+    c.abort_host = nullptr;
+    c.abort_dev = nullptr;
+    c.perft_total_host = nullptr;
+    (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+    return rc ? rc : fail(ZETA_ENODEV, "hipEventCreate: %s", hipGetErrorString(e));
+  }
   c.cfg = d;
   c.device = d.device;
   memset(c.kstat, 0, sizeof(c.kstat));
@@ -185,9 +204,12 @@ extern "C" int zeta_shutdown(void) {
   mlp_release();
   for (auto e : c.event_pool) (void)hipEventDestroy(e);
   c.event_pool.clear();
+  if (c.ev_total) (void)hipEventDestroy(c.ev_total);
+  c.ev_total = nullptr;
   if (c.abort_host) (void)hipHostFree(c.abort_host);
   c.abort_host = nullptr;
   c.abort_dev = nullptr;
+  c.perft_total_host = nullptr;
   (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
   c.up = false;
