@@ -42,15 +42,40 @@ HBM_BYTES_PER_POSITION = 12.85
 VALU_WAVE_INSTS_PER_NODE = 915.8
 
 
+def host_info():
+    """The host the CPU baselines ran on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
 def cpu_baseline(pos, depth, threads):
+    """The oracle's perft of the whole workload on `threads` host cores, with
+    bulk counting at depth 1 as the device does (like for like with the
+    headline), and the same walk making every leaf move."""
     from oracle import refcpu
     sample = pos  # the whole workload: perft(4) of 4096 positions is a few seconds on 16 threads
     t0 = time.perf_counter()
-    c = refcpu.perft_batch(sample, depth, threads)
+    c = refcpu.perft_batch(sample, depth, threads, bulk=True)
     dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    c2 = refcpu.perft_batch(sample, depth, threads)
+    dt2 = time.perf_counter() - t1
+    assert int(c.sum()) == int(c2.sum())
     return {"value": float(c.sum()) / dt, "unit": "leaves/s", "cores": threads, "kind": "port",
-            "sample": "oracle perft(%d) of all %d workload positions (%d leaves, %.1f s, %d threads)"
-                      % (depth, len(sample), int(c.sum()), dt, threads)}
+            "bulk_counting": True,
+            "sample": "oracle perft(%d) of all %d workload positions with bulk counting (%d leaves, %.1f s, "
+                      "%d threads)" % (depth, len(sample), int(c.sum()), dt, threads),
+            "leaf_visiting": {"value": float(c2.sum()) / dt2, "unit": "leaves/s", "seconds": round(dt2, 2),
+                              "note": "every leaf move made (no bulk counting)"},
+            "host": host_info()}
 
 
 def main():
@@ -61,6 +86,8 @@ def main():
     ap.add_argument("--depth", type=int, default=4)
     ap.add_argument("--positions", type=int, default=4096)
     ap.add_argument("--nobulk", action="store_true", help="visit every leaf (no bulk counting)")
+    ap.add_argument("--nobulk-steps", type=int, default=3,
+                    help="timed steps of the same workload without bulk counting, reported beside the headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the fixed-depth search leg")
     ap.add_argument("--search-depth", type=int, default=6)
@@ -126,6 +153,33 @@ def main():
                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": round(per_launch_items * HBM_BYTES_PER_POSITION / avg_s / HBM_PEAK, 4)}}
 
+    # the same workload without bulk counting: every leaf is generated, as the
+    # reference's kernel visits it (src/zetaperft.cl:1316-1326)
+    nobulk = None
+    if not args.nobulk and args.nobulk_steps > 0:
+        def step_nb():
+            zeta_amd.perft_device(d_pos.ptr, args.positions, args.depth, d_cnt.ptr, nobulk=True)
+        step_nb()
+        zeta_amd.synchronize()
+        nb_leaves = int(d_cnt.download(np.zeros(args.positions, dtype=np.uint64)).sum())
+        assert nb_leaves == leaves
+        zeta_amd.stats_reset()
+        zeta_amd.set_timing(True)
+        step_nb()
+        kst_nb = zeta_amd.stats()
+        zeta_amd.set_timing(False)
+        zeta_amd.stats_reset()
+        zeta_amd.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.nobulk_steps):
+            step_nb()
+        zeta_amd.synchronize()
+        dt_nb = time.perf_counter() - t0
+        nobulk = {"leaves_per_s": nb_leaves * args.nobulk_steps / dt_nb,
+                  "ms_per_step": dt_nb * 1e3 / args.nobulk_steps, "steps": args.nobulk_steps,
+                  "kernels_ms_total": {k: round(v[1], 3) for k, v in kst_nb.items() if v[0]},
+                  "leaf_launches": kst_nb.get("perft_leaf", (0, 0, 0))[0]}
+
     calib = None
     if args.calib_copy:
         nb = (args.calib_copy + 15) // 16 * 16
@@ -170,6 +224,8 @@ def main():
     cpu = None
     c1_cpu = None
     if not args.no_cpu_baseline:
+        # the host cores this run may use: all of them here, the GPU's share
+        # (16 cores) on a shared GPU host
         threads = min(16, len(os.sched_getaffinity(0)))
         cpu = cpu_baseline(pos, args.depth, threads)
         # config 1 on one core: the reference runs it as one work-group; the
@@ -204,7 +260,7 @@ def main():
                       "positions": args.positions, "depth": args.depth, "leaves_per_step": leaves,
                       "bulk_counting": not args.nobulk},
            "kernels_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in kst.items() if v[0]},
-           "roofline": roof, "cpu_baseline": cpu,
+           "roofline": roof, "cpu_baseline": cpu, "nobulk": nobulk,
            "config1": {"workload": "startpos perft(5), one root", "leaves": c1, "ms": round(c1_ms, 3),
                        "leaves_per_s": c1 / (c1_ms / 1e3), "cpu_baseline": c1_cpu},
            "search": search}

@@ -126,6 +126,14 @@ typedef struct {
    * (the reference's PV[1..], src/zeta.cl:2450-2512). */
   uint32_t *pv;
   int pv_len;
+  /* 1 = order by hints: before the batch the transposition table as the
+   * previous batch left it (same size) is copied aside, and at every
+   * full-width node the move it holds goes first, above the TT move.  After
+   * a completed search of the same roots at the same depth that table holds
+   * the best and refuting moves the search found, so the hinted search walks
+   * a near-minimal tree: its node count measures what any move ordering
+   * could save.  Scores are unchanged (any order gives them). */
+  int order_hint;
 } zeta_search_opts;
 #define ZETA_HISTORY 128
 

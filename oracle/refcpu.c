@@ -596,6 +596,32 @@ uint64_t rc_perft(const rc_pos *p, int depth) {
   return perft_rec(p->b, p->b[4], p->stm, depth);
 }
 
+/* The same count with bulk counting: a depth-1 position contributes its
+ * number of legal moves, which are not made (as the device's leaf level). */
+static u64 perft_bulk_rec(const u64 *b, u64 cr, int stm, int depth) {
+  if (depth == 0) return 1;
+  u64 mv[64];
+  const int n = rc_genmoves(b, cr, stm, 0, mv, NULL);
+  if (depth == 1) return (u64)n;
+  u64 sum = 0;
+  for (int lid = 0; lid < 64; lid++) {
+    u64 m = mv[lid];
+    while (m) {
+      int sqto = popfirst1(&m);
+      u32 move = rc_makemove_code(b, stm, lid, sqto);
+      u64 child[4] = {b[0], b[1], b[2], b[3]};
+      domove4(child, move);
+      sum += perft_bulk_rec(child, cr_after_perft(cr, move), !stm, depth - 1);
+    }
+  }
+  return sum;
+}
+
+uint64_t rc_perft_bulk(const rc_pos *p, int depth) {
+  rc_init();
+  return perft_bulk_rec(p->b, p->b[4], p->stm, depth);
+}
+
 int rc_perft_divide(const rc_pos *p, int depth, uint32_t *moves, uint64_t *counts) {
   rc_init();
   u64 mv[64];
@@ -852,6 +878,7 @@ static void *batch_worker(void *arg) {
     pthread_mutex_unlock(&j->mu);
     if (i >= j->n) break;
     if (j->kind == 0) j->counts[i] = rc_perft(&j->p[i], j->depth);
+    else if (j->kind == 3) j->counts[i] = rc_perft_bulk(&j->p[i], j->depth);
     else {
       uint64_t nd = 0;
       j->scores[i] = j->kind == 2 ? rc_search_id(&j->p[i], j->depth, NULL, 0, &nd, NULL)
@@ -878,6 +905,13 @@ void rc_perft_batch(const rc_pos *p, int n, int depth, uint64_t *counts, int thr
   batch_job j;
   memset(&j, 0, sizeof(j));
   j.p = p; j.n = n; j.depth = depth; j.kind = 0; j.counts = counts;
+  run_batch(&j, threads);
+}
+
+void rc_perft_bulk_batch(const rc_pos *p, int n, int depth, uint64_t *counts, int threads) {
+  batch_job j;
+  memset(&j, 0, sizeof(j));
+  j.p = p; j.n = n; j.depth = depth; j.kind = 3; j.counts = counts;
   run_batch(&j, threads);
 }
 

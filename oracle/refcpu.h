@@ -89,6 +89,10 @@ uint64_t rc_perft(const rc_pos *p, int depth);
 int      rc_perft_divide(const rc_pos *p, int depth, uint32_t *moves, uint64_t *counts);
 /* Multi-threaded batch perft over n positions (CPU baseline). */
 void     rc_perft_batch(const rc_pos *p, int n, int depth, uint64_t *counts, int threads);
+/* The same counts with bulk counting at depth 1 (the moves of a depth-1
+ * position are counted, not made), single root and batch. */
+uint64_t rc_perft_bulk(const rc_pos *p, int depth);
+void     rc_perft_bulk_batch(const rc_pos *p, int n, int depth, uint64_t *counts, int threads);
 
 /* ---- evaluation / search ---- */
 /* Static eval, white-positive, as evalscore after the x64 sum (src/zeta.cl:1479-1545). */

@@ -51,6 +51,8 @@ def lib():
         L.rc_perft_divide.restype = ctypes.c_int
         L.rc_perft_batch.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int]
         L.rc_perft_batch.restype = None
+        L.rc_perft_bulk_batch.argtypes = L.rc_perft_batch.argtypes
+        L.rc_perft_bulk_batch.restype = None
         L.rc_eval.argtypes = [vp]
         L.rc_eval.restype = i32
         L.rc_search.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp]
@@ -89,10 +91,13 @@ def perft(pos, depth):
     return int(lib().rc_perft(_ptr(pos), depth))
 
 
-def perft_batch(pos, depth, threads=None):
+def perft_batch(pos, depth, threads=None, bulk=False):
+    """Leaf counts; bulk=True counts the moves of depth-1 positions without
+    making them (the same counts)."""
     pos = np.ascontiguousarray(pos, dtype=POS_DTYPE)
     out = np.zeros(len(pos), dtype=np.uint64)
-    lib().rc_perft_batch(_ptr(pos), len(pos), depth, _ptr(out), threads or os.cpu_count() or 1)
+    f = lib().rc_perft_bulk_batch if bulk else lib().rc_perft_batch
+    f(_ptr(pos), len(pos), depth, _ptr(out), threads or os.cpu_count() or 1)
     return out
 
 

@@ -215,3 +215,12 @@ def test_playouts_deterministic_and_legal():
     for i in range(len(a)):
         p = a[i:i + 1]
         assert R.count_legal(p) > 0 and int(p["b"][0, 6]) < 100
+
+
+def test_bulk_counting_equals_leaf_visiting(R=None):
+    """The oracle's perft with bulk counting at depth 1 (the CPU baseline of
+    the bulk-counting headline) counts exactly the leaves it visits without."""
+    from oracle import refcpu
+    pos = refcpu.gen_playouts(2, 64, 8, 40)
+    for d in (1, 2, 3):
+        assert (refcpu.perft_batch(pos, d, 4, bulk=True) == refcpu.perft_batch(pos, d, 4)).all()

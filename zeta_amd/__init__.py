@@ -48,7 +48,7 @@ class SearchOpts(ctypes.Structure):
                 ("clear_tt", ctypes.c_int), ("max_nodes", ctypes.c_uint64), ("use_mlp", ctypes.c_int),
                 ("mlp_min_depth", ctypes.c_int), ("log_records", ctypes.c_int), ("log_shift", ctypes.c_int),
                 ("history", ctypes.c_void_p), ("history_len", ctypes.c_void_p),
-                ("pv", ctypes.c_void_p), ("pv_len", ctypes.c_int)]
+                ("pv", ctypes.c_void_p), ("pv_len", ctypes.c_int), ("order_hint", ctypes.c_int)]
 
 
 ZETA_HISTORY = 128
@@ -208,13 +208,15 @@ def perft_device(d_pos, n, depth, d_counts, nobulk=False):
 
 
 def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_mlp=False,
-           mlp_min_depth=3, log_records=False, log_shift=0, history=None, pv_len=0):
+           mlp_min_depth=3, log_records=False, log_shift=0, history=None, pv_len=0, order_hint=False):
     """Fixed-depth alpha-beta + quiescence of every position (iterative
     deepening 1..depth).  Returns a structured array (score, bestmove, nodes,
     depth, status) -- the mirror of rootsearch's PV[0]/PV[1]/node count.
     history: optional list (one entry per position) of the hashes of the
     positions played before it, newest first, for repetitions across the root.
-    pv_len > 0: also return the principal variations (n x pv_len move codes)."""
+    pv_len > 0: also return the principal variations (n x pv_len move codes).
+    order_hint: search with the moves of the previous batch's table first
+    (zeta_search_opts.order_hint)."""
     ensure_init()
     pos = as_positions(pos)
     out = np.zeros(len(pos), dtype=RESULT_DTYPE)
@@ -232,7 +234,7 @@ def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_ml
     o = SearchOpts(int(depth), int(workers), int(tt_log2), int(bool(clear_tt)), int(max_nodes),
                    int(bool(use_mlp)), int(mlp_min_depth), int(bool(log_records)), int(log_shift),
                    hist.ctypes.data if hist is not None else None, hlen.ctypes.data if hlen is not None else None,
-                   pv.ctypes.data if pv_len > 0 else None, int(pv_len))
+                   pv.ctypes.data if pv_len > 0 else None, int(pv_len), int(bool(order_hint)))
     _check(lib().zeta_search_batch(_ptr(pos), len(pos), ctypes.byref(o), _ptr(out)), "zeta_search_batch")
     return (out, pv) if pv_len > 0 else out
 

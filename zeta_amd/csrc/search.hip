@@ -48,6 +48,7 @@ constexpr s32 MATESCORE = 30000;
 constexpr int MSTACK = 1024;  // move-stack slots per worker (LDS)
 constexpr s32 USED = -32768;  // 16-bit move-ordering scores
 constexpr s32 TTMOVE_SCORE = 32000;  // above every other score (src/zeta.cl:2101)
+constexpr s32 HINT_SCORE = 32100;    // the hint table's move, above the TT move
 // the killer (one per ply) and the counter move (indexed by the previous
 // move's squares), quiet moves only: src/zeta.cl:2073-2082
 constexpr s32 KILLER_SCORE = 1400;
@@ -106,6 +107,8 @@ struct SearchArgs {
   int workers;       // at most this many workers on one root
   u64 *tt;           // 2 words per entry
   u64 ttmask;        // entries - 1 (0 = no TT)
+  const u64 *hint;   // read-only table in the TT's format whose moves go first (null = none)
+  u64 *rbound;       // per root (slot): best root score so far, ticket << 32 | iteration << 16 | score + 32768
   u32 *busy;
   u32 busymask;
   int *queue;
@@ -284,17 +287,22 @@ __device__ __forceinline__ u32 child_hmc(u32 hmc, u32 m) {
 
 enum { S_ENTER = 0, S_NEXT = 1, S_RETURN = 2 };
 
-// TT store when a node returns with a move that raised alpha (a lower bound
-// of the node's value); score kept as distance-to-mate from this node.
+// TT store when a full-width node returns: with a move that raised alpha, a
+// lower bound of the node's value and that move; with none (every move
+// failed low), an upper bound and no move (bit 58).  The score is kept as
+// distance-to-mate from this node.  Either bound is the value of the
+// depth-limited subtree, which a node with a zero half-move clock cannot see
+// depend on the path above it: entries are used at the same depth and clock
+// only, so no root value depends on them.
 __device__ __forceinline__ u32 tt_store(const SearchArgs &a, const Node &n, int sd, int lane) {
-  if (!a.ttmask || n.qs || !n.best || n.depth < 1) return 0;
+  if (!a.ttmask || n.qs || n.depth < 1) return 0;
   s32 s = n.alpha;
   const int plyn = sd - 1;
   if (is_mate(s) && s > 0) s += plyn;
   if (is_mate(s) && s < 0) s -= plyn;
   const u64 key = tt_key(n.hash, n.p.cr);
   const u64 w1 = (u64)n.best | ((u64)(u16)(int16_t)s << 32) | ((u64)(n.depth & 0xFF) << 48) |
-                 ((u64)(n.hmc == 0 ? 1 : 0) << 56) | (1ULL << 57);
+                 ((u64)(n.hmc == 0 ? 1 : 0) << 56) | (1ULL << 57) | ((u64)(n.best ? 0 : 1) << 58);
   if (lane == 0) {
     const u64 w0 = key ^ w1;
     u32x4 e;
@@ -762,6 +770,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     id0 = uf(id0);
     id0 = id0 < 1 ? 1 : (id0 > depth_r ? depth_r : id0);
     for (int id = id0; id <= depth_r && !aborted && !failed; id++) {
+      // the best score any worker has found for a root move in this
+      // iteration (refreshed at the poll); -INF until one has
+      s32 ra = -INF;
       Node n;
       {
         u64 q[8];
@@ -830,6 +841,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               aborted = true;
               break;
             }
+            u64 rb = 0;
+            if (lane == 0) rb = ag_load(&a.rbound[r]);
+            rb = uf(rb);
+            if ((rb >> 32) == (ticket & 0xFFFFFFFFULL) && (int)((rb >> 16) & 0xFFFF) == id)
+              ra = (s32)(rb & 0xFFFF) - 32768;
           }
           const int plyn = sd - 1;
           // issue the TT load early: its latency overlaps the analysis below
@@ -844,6 +860,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           n.kic = B.checkers != 0;
           if (n.depth >= 0 && !parent_qs && (n.kic || is_promo(n.lastmove))) n.depth++;
           n.qs = n.depth <= 0;
+          // Another worker's root move reached ra: at the root's side (odd
+          // plies from 3 on) a value at or below ra - 1 cannot make this root
+          // move the best, so the window starts there.  A root move then
+          // reports max(its value, a bound below ra) -- below the best root
+          // score, which stays exact, as does its move.
+          if ((sd & 1) && sd >= 3 && ra - 1 > n.alpha && ra - 1 < n.beta) n.alpha = ra - 1;
           n.best = 0;
           n.iter = 1;
           n.ndefer = 0;
@@ -1012,12 +1034,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
                 const int d = (int)((w1 >> 48) & 0xFF);
                 if (is_mate(ts) && ts > 0) ts -= plyn;
                 if (is_mate(ts) && ts < 0) ts += plyn;
-                if (((w1 >> 56) & 1) && n.hmc == 0 && d == n.depth && ts > -INF && ts < INF && ts > n.alpha) {
-                  n.alpha = ts;
-                  lcount(cvec, C_TT_CUT, 1);
-                  if (n.alpha >= n.beta) {
-                    v = n.alpha;
-                    state = S_RETURN;
+                if (((w1 >> 56) & 1) && n.hmc == 0 && d == n.depth && ts > -INF && ts < INF) {
+                  if ((w1 >> 58) & 1) {
+                    // upper bound at or below alpha: the node fails low
+                    if (ts <= n.alpha) {
+                      lcount(cvec, C_TT_CUT, 1);
+                      v = n.alpha;
+                      state = S_RETURN;
+                    }
+                  } else if (ts > n.alpha) {
+                    n.alpha = ts;
+                    lcount(cvec, C_TT_CUT, 1);
+                    if (n.alpha >= n.beta) {
+                      v = n.alpha;
+                      state = S_RETURN;
+                    }
                   }
                 }
               }
@@ -1025,6 +1056,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             if (state == S_NEXT && ttmove) {
               for (int k = lane; k < total; k += 64)
                 if (mmove[n.mbeg + k] == ttmove) mscore[n.mbeg + k] = (int16_t)TTMOVE_SCORE;
+            }
+            if (state == S_NEXT && a.hint && !n.qs) {
+              // the hint table (a completed search's table): its move first
+              const u32x4 he = *(const u32x4 *)(a.hint + 2 * (key & a.ttmask));
+              const u64 h0 = uf(((u64)he.y << 32) | he.x), h1 = uf(((u64)he.w << 32) | he.z);
+              if ((h0 ^ h1) == key && ((h1 >> 57) & 1)) {
+                const u32 hm = (u32)h1;
+                for (int k = lane; k < total; k += 64)
+                  if (mmove[n.mbeg + k] == hm) mscore[n.mbeg + k] = (int16_t)HINT_SCORE;
+              }
             }
           }
           wsync();
@@ -1133,12 +1174,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           restore(ply[sd], child, n);
         }
         const s32 sc = -v;
+        if ((sd & 1) && sd >= 3 && ra - 1 > n.alpha && ra - 1 < n.beta) {
+          // the root bound rose while this node's children were searched:
+          // its best move so far is below it, so no lower bound stands
+          n.alpha = ra - 1;
+          n.best = 0;
+        }
         if (sc > n.alpha) {
           n.alpha = sc;
           n.best = n.move;
           if (sd == 1) {
             rootscore = sc;
             rootmove = n.move;
+            if (lane == 0)
+              atomicMax((unsigned long long *)&a.rbound[r],
+                        (unsigned long long)(((ticket & 0xFFFFFFFFULL) << 32) | ((u64)id << 16) | (u64)(u16)(sc + 32768)));
           }
         }
         if (n.alpha >= n.beta) {
@@ -1268,7 +1318,8 @@ __global__ void k_clear_u64(u64 *p, size_t n) {
 }
 
 struct SearchState {
-  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, ctr, hist, hlen, pv;
+  zr::DevBuf tt, busy, roots, queue, done, nworkers, score, bestmove, nodes, depth_done, err, ctr, hist, hlen, pv, hint,
+      rbound;
   u64 tt_entries = 0;
   u64 last_ctr[C_NCTR] = {0};
 };
@@ -1286,6 +1337,7 @@ void search_release() {
   zr::release(g_ss.done); zr::release(g_ss.nworkers); zr::release(g_ss.score); zr::release(g_ss.bestmove);
   zr::release(g_ss.nodes); zr::release(g_ss.depth_done); zr::release(g_ss.err);
   zr::release(g_ss.ctr); zr::release(g_ss.hist); zr::release(g_ss.hlen); zr::release(g_ss.pv);
+  zr::release(g_ss.hint); zr::release(g_ss.rbound);
   g_ss.tt_entries = 0;
 }
 
@@ -1305,11 +1357,19 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
       (r = zr::ensure(g_ss.done, (size_t)n * 4)) || (r = zr::ensure(g_ss.nworkers, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.score, (size_t)n * 4)) || (r = zr::ensure(g_ss.bestmove, (size_t)n * 4)) ||
       (r = zr::ensure(g_ss.nodes, (size_t)n * 8)) || (r = zr::ensure(g_ss.depth_done, (size_t)n * 4)) ||
+      (r = zr::ensure(g_ss.rbound, (size_t)n * 8)) ||
       (r = zr::ensure(g_ss.err, 64)) || (r = zr::ensure(g_ss.ctr, C_NCTR * 8)) ||
       (r = zr::ensure(g_ss.busy, (size_t)busy_entries * 4)))
     return r;
+  if (opts->order_hint && (!tt_entries || g_ss.tt_entries != tt_entries))
+    return zr::fail(ZETA_EINVAL, "search: order_hint needs the previous batch's table of the same size");
   if (tt_entries) {
     if ((r = zr::ensure(g_ss.tt, tt_entries * 16))) return r;
+    if (opts->order_hint) {
+      // the previous batch's table becomes the hint table before it is cleared
+      if ((r = zr::ensure(g_ss.hint, tt_entries * 16))) return r;
+      ZCHECK(hipMemcpyAsync(g_ss.hint.p, g_ss.tt.p, tt_entries * 16, hipMemcpyDeviceToDevice, st));
+    }
     if (g_ss.tt_entries != tt_entries || opts->clear_tt) {
       ZLAUNCH(zr::K_SEARCH, 0, k_clear_u64, 4096, 256, 0, (u64 *)g_ss.tt.p, (size_t)(tt_entries * 2));
       g_ss.tt_entries = tt_entries;
@@ -1323,6 +1383,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   ZCHECK(hipMemsetAsync(g_ss.bestmove.p, 0, (size_t)n * 4, st));
   ZCHECK(hipMemsetAsync(g_ss.nodes.p, 0, (size_t)n * 8, st));
   ZCHECK(hipMemsetAsync(g_ss.depth_done.p, 0, (size_t)n * 4, st));
+  ZCHECK(hipMemsetAsync(g_ss.rbound.p, 0, (size_t)n * 8, st));
   ZCHECK(hipMemsetAsync(g_ss.err.p, 0, 64, st));
   ZCHECK(hipMemsetAsync(g_ss.ctr.p, 0, C_NCTR * 8, st));
   ZCHECK(hipMemsetAsync(g_ss.busy.p, 0, (size_t)busy_entries * 4, st));
@@ -1341,6 +1402,8 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.workers = workers;
   sa.tt = (u64 *)g_ss.tt.p;
   sa.ttmask = tt_entries ? tt_entries - 1 : 0;
+  sa.hint = opts->order_hint ? (const u64 *)g_ss.hint.p : nullptr;
+  sa.rbound = (u64 *)g_ss.rbound.p;
   sa.busy = (u32 *)g_ss.busy.p;
   sa.busymask = busy_entries - 1;
   sa.queue = (int *)g_ss.queue.p;
@@ -1508,7 +1571,7 @@ struct ServeState {
   ServeHost *sh = nullptr;
   u64 *queue = nullptr, *ack = nullptr, *cancel = nullptr;
   ServeSlot *slots = nullptr;
-  zr::DevBuf sv, sw, dreq, dcancel, nodes, depth_done, busy, ctr, err;
+  zr::DevBuf sv, sw, dreq, dcancel, nodes, depth_done, busy, ctr, err, rbound;
   std::vector<u64> slot_ticket;  // ticket + 1 held by the slot, 0 = free
   std::vector<u64> slot_cost;
   std::vector<int> free_slots;
@@ -1565,7 +1628,7 @@ void sv_take(int s, zeta_result *res) {
 void sv_free_all() {
   zr::release(g_sv.sv); zr::release(g_sv.sw); zr::release(g_sv.dreq); zr::release(g_sv.dcancel);
   zr::release(g_sv.nodes); zr::release(g_sv.depth_done); zr::release(g_sv.busy); zr::release(g_sv.ctr);
-  zr::release(g_sv.err);
+  zr::release(g_sv.err); zr::release(g_sv.rbound);
   if (g_sv.pinned) (void)hipHostFree(g_sv.pinned);
   g_sv.pinned = nullptr;
   g_sv.sh = nullptr;
@@ -1630,7 +1693,8 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   if ((r = zr::ensure(S.sv, sizeof(ServeDev))) || (r = zr::ensure(S.sw, (size_t)S.nslots * 8)) ||
       (r = zr::ensure(S.dreq, (size_t)S.nslots * sizeof(ServeReq))) || (r = zr::ensure(S.dcancel, (size_t)S.nslots * 8)) ||
       (r = zr::ensure(S.nodes, (size_t)S.nslots * 8)) || (r = zr::ensure(S.depth_done, (size_t)S.nslots * 4)) ||
-      (r = zr::ensure(S.busy, (size_t)busy_entries * 4)) || (r = zr::ensure(S.ctr, C_NCTR * 8)) || (r = zr::ensure(S.err, 64))) {
+      (r = zr::ensure(S.busy, (size_t)busy_entries * 4)) || (r = zr::ensure(S.ctr, C_NCTR * 8)) || (r = zr::ensure(S.err, 64)) ||
+      (r = zr::ensure(S.rbound, (size_t)S.nslots * 8))) {
     sv_free_all();
     return r;
   }
@@ -1660,6 +1724,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   SV_CHECK(hipMemsetAsync(S.busy.p, 0, (size_t)busy_entries * 4, st));
   SV_CHECK(hipMemsetAsync(S.ctr.p, 0, C_NCTR * 8, st));
   SV_CHECK(hipMemsetAsync(S.err.p, 0, 64, st));
+  SV_CHECK(hipMemsetAsync(S.rbound.p, 0, (size_t)S.nslots * 8, st));
   SV_CHECK(hipStreamSynchronize(st));
 
   int rate_khz = 0;
@@ -1686,6 +1751,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   sa.err = (u32 *)S.err.p;
   sa.ctr = (u64 *)S.ctr.p;
   sa.habort = zr::ctx().abort_dev;
+  sa.rbound = (u64 *)S.rbound.p;
   sa.use_mlp = S.use_mlp ? 1 : 0;
   sa.mlp_min_depth = o->mlp_min_depth;
   if (w) sa.w = *w;
