@@ -1,7 +1,9 @@
 """CPU oracle of the learned move-ordering network -- TEST INFRASTRUCTURE ONLY.
 
 numpy restatement of the network the GPU trains (908 -> 256 -> 32 -> 1,
-ReLU, pairwise ranking loss softplus(s_neg - s_pos)), in two forms:
+ReLU, pairwise ranking loss softplus(s_neg - s_pos + o / PLAIN_PER_LOGIT),
+o the plain ordering's score difference the pair carries in the upper 24
+bits of its depth word), in two forms:
   * forward_backward(..., bf16=True): FP32 arithmetic with BF16 rounding at
     the points where the GPU feeds BF16 MFMA operands (weights W1, W2,
     activations h1, and the gradients dz2, dz1): the values the kernels'
@@ -17,6 +19,16 @@ import numpy as np
 NIN, NH1, NH2 = 908, 256, 32
 NPARAMS = NIN * NH1 + NH1 + NH1 * NH2 + NH2 + NH2 + 1  # 240961
 PAIR_DTYPE = np.dtype([("b", "<u8", (4,)), ("stm", "<u4"), ("mpos", "<u4"), ("mneg", "<u4"), ("depth", "<u4")])
+PLAIN_PER_LOGIT = 100
+
+
+def pair_offsets(pairs):
+    """Plain score difference (negative minus positive) of each pair."""
+    return (np.asarray(pairs["depth"], dtype=np.uint32).view(np.int32) >> 8).astype(np.float64)
+
+
+def depth_word(depth, offset):
+    return np.uint32(((int(offset) << 8) | (int(depth) & 0xFF)) & 0xFFFFFFFF)
 
 
 def bf16(x):
@@ -90,7 +102,7 @@ def forward_backward(params, pairs, bf16_points=True):
         z2 = h1b @ W2b + b2.astype(f)
         h2 = np.maximum(z2, 0).astype(f)
         s = h2 @ W3.astype(f) + f(b3[0])
-        d = s[P:] - s[:P]
+        d = s[P:] - s[:P] + (pair_offsets(pairs) / PLAIN_PER_LOGIT).astype(f)
         sig = 1 / (1 + np.exp(-d))
         loss = float(np.mean(np.logaddexp(0, d)))
         ds = np.concatenate([-sig, sig]).astype(f) / f(P)
@@ -110,7 +122,7 @@ def forward_backward(params, pairs, bf16_points=True):
         z2 = h1 @ W2 + b2
         h2 = np.maximum(z2, 0)
         s = h2 @ W3 + b3[0]
-        d = s[P:] - s[:P]
+        d = s[P:] - s[:P] + pair_offsets(pairs) / PLAIN_PER_LOGIT
         sig = 1 / (1 + np.exp(-d))
         loss = float(np.mean(np.logaddexp(0, d)))
         ds = np.concatenate([-sig, sig]) / P

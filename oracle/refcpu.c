@@ -835,6 +835,11 @@ static s32 search_common(const rc_pos *p, int depth, const uint64_t *hist, int p
   for (int d = id ? 1 : depth; d <= depth; d++) {
     c->bestscore = 0; /* DRAWSCORE until a root move raises alpha (src/zeta.cl:986) */
     ab(c, p->b, p->b[4], p->b[5], (u8)p->b[6], p->stm, d, -RC_INF, RC_INF, 1, 0, 0);
+    /* mate in n: a mate within this iteration's depth ends the iterations
+     * (src/search.c:177-186) */
+    const s32 bs = c->bestscore;
+    if (id && bs > RC_MATESCORE && bs < RC_INF && d >= RC_INF - bs) break;
+    if (id && bs < -RC_MATESCORE && bs > -RC_INF && d >= RC_INF + bs) break;
   }
   s32 sc = c->bestscore;
   if (sc == RC_INF || sc == -RC_INF) sc = 0; /* src/search.c:137 */

@@ -105,7 +105,8 @@ int32_t  rc_eval(const uint64_t *board4);
 int32_t  rc_search(const rc_pos *p, int depth, const uint64_t *hist, int ply_init,
                    uint64_t *nodes, uint32_t *bestmove);
 /* Iterative deepening 1..depth of rc_search (src/search.c:77-198), the
- * killers and counter moves kept across the iterations: the score of the
+ * killers and counter moves kept across the iterations and a mate within an
+ * iteration's depth ending them (src/search.c:177-186): the score of the
  * last iteration and the moves made over all of them -- the node count a
  * single-worker device search without a transposition table must equal. */
 int32_t  rc_search_id(const rc_pos *p, int depth, const uint64_t *hist, int ply_init,

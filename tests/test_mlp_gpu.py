@@ -25,6 +25,7 @@ def _pairs(oracle, n, seed):
         out[i]["stm"] = int(p["stm"][0])
         out[i]["mpos"] = oracle.nth_legal(p, int(a))
         out[i]["mneg"] = oracle.nth_legal(p, int(b))
+        out[i]["depth"] = M.depth_word(1 + i % 6, int(rng.integers(-400, 400)))
         i += 1
         if i == n:
             break

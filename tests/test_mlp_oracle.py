@@ -8,6 +8,7 @@ from oracle import refcpu as R
 
 def make_pairs(n, seed=0):
     rng = np.random.default_rng(seed)
+    orng = np.random.default_rng(1000 + seed)
     pos = R.gen_playouts(100 + seed, n, 8, 40)
     out = np.zeros(n, dtype=M.PAIR_DTYPE)
     for i in range(n):
@@ -18,6 +19,8 @@ def make_pairs(n, seed=0):
         out[i]["stm"] = int(p["stm"][0])
         out[i]["mpos"] = R.nth_legal(p, int(a))
         out[i]["mneg"] = R.nth_legal(p, int(b))
+        # the plain ordering's score difference the search logs with a pair
+        out[i]["depth"] = M.depth_word(1 + i % 6, int(orng.integers(-400, 400)))
     return out
 
 
@@ -57,8 +60,24 @@ def test_finite_differences_float64():
         assert abs(num - g[i]) <= 1e-5 + 1e-4 * abs(num), (i, num, g[i])
 
 
+def test_plain_offset_closed_form():
+    """With the network constant (all weights zero) every pair's loss is
+    softplus(o / PLAIN_PER_LOGIT): the plain ordering's score difference o
+    (negative minus positive) counts against the pair in plain-score units."""
+    pairs = make_pairs(5, seed=7)
+    offs = [int(x) for x in (-250, -1, 0, 37, 900)]
+    for i, o in enumerate(offs):
+        pairs[i]["depth"] = M.depth_word(3, o)
+    assert [int(v) for v in M.pair_offsets(pairs)] == offs
+    loss, g = M.forward_backward(np.zeros(M.NPARAMS), pairs, bf16_points=False)
+    want = np.mean([np.logaddexp(0, o / 100.0) for o in offs])
+    assert abs(loss - want) < 1e-12
+    assert np.count_nonzero(g[:-33]) == 0  # relu(0) = 0: only the head moves
+
+
 def test_bf16_variant_close_to_exact():
     pairs = make_pairs(32, seed=3)
+    pairs["depth"] = 3  # no plain offsets: no pair saturates, the BF16 rounding dominates the difference
     w = random_params(4, 0.1)
     l64, g64 = M.forward_backward(w, pairs, bf16_points=False)
     l16, g16 = M.forward_backward(w.astype(np.float32), pairs, bf16_points=True)

@@ -67,7 +67,10 @@ def test_serve_mix_equals_oracle_and_alone(zeta, mixed):
         print("serve: mixed", list(scores), flush=True)
         assert (status == 0).all()
         assert (scores == want).all()
-        assert all(int(g["depth"]) == int(d) for g, d in zip(got, depths))
+        # every iteration up to the requested depth, unless a mate within
+        # an earlier iteration's depth ended them (src/search.c:177-186)
+        assert all(int(g["depth"]) == int(d) or (abs(int(g["score"])) > 30000 and int(g["depth"]) <= int(d))
+                   for g, d in zip(got, depths))
         print("serve: each request alone", flush=True)
         for k in range(8):
             t = sv.submit(pos[k], int(depths[k]))

@@ -31,6 +31,15 @@ constexpr int P_B2 = P_W2 + NH1 * NH2;    // 32
 constexpr int P_W3 = P_B2 + NH2;          // 32
 constexpr int P_B3 = P_W3 + NH2;          // 1
 
+// The search orders moves by plain score + PLAIN_PER_LOGIT x network output:
+// the network learns a correction of the plain ordering.  A training pair
+// carries the plain score difference (negative minus positive) in the upper
+// 24 bits of its depth word; the loss adds it / PLAIN_PER_LOGIT.
+constexpr int PLAIN_PER_LOGIT = 100;
+constexpr int PAIR_OFFSET_MAX = (1 << 23) - 1;
+ZD u32 pair_depth_word(int depth, int offset) { return ((u32)offset << 8) | ((u32)depth & 0xFFu); }
+ZD int pair_offset(u32 w) { return (int)w >> 8; }
+
 // device weight set used by the search and by training
 struct Weights {
   const u16 *w1f;   // BF16 [928][256]: W1, unit-contiguous (row gathers in the search)

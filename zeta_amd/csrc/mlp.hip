@@ -5,9 +5,11 @@
 //
 // Training examples are pairs logged by the plain search at cut nodes: the
 // move that caused the beta cutoff and a move searched before it that did
-// not.  A step takes P pairs = B = 2P examples (rows [0,P) the cutoff
-// moves, rows [P,2P) the failed moves) and minimises
-//   L = mean_p softplus(s(neg_p) - s(pos_p)).
+// not (or one never searched).  A step takes P pairs = B = 2P examples (rows
+// [0,P) the cutoff moves, rows [P,2P) the others) and minimises
+//   L = mean_p softplus(s(neg_p) - s(pos_p) + o_p / PLAIN_PER_LOGIT),
+// o_p the plain ordering's score difference of the pair: the search ranks
+// by plain score + PLAIN_PER_LOGIT x s, so s learns a correction of it.
 // All five products are one NT GEMM kernel, C[M][N] = sum_k A[m][k] Bt[n][k],
 // both operands BF16 with k contiguous, one 32x32 C tile per wavefront built
 // from v_mfma_f32_32x32x16_bf16:
@@ -117,6 +119,7 @@ __global__ void k_l1_act(float *Z1, const float *__restrict__ b1, u16 *H1, u16 *
 // layer 2 activation, layer 3, pairwise ranking loss and the gradients that
 // need no GEMM; one thread per pair (rows p and P+p)
 __global__ __launch_bounds__(256) void k_head(const float *__restrict__ Z2, const float *__restrict__ prm, int P,
+                                              const PairRec *__restrict__ recs, const u32 *__restrict__ idx,
                                               u16 *dZ2, u16 *dZ2t, float *grad, float *loss_out) {
   __shared__ float red[NH2 * 2 + NH2 + 2];
   for (int i = threadIdx.x; i < NH2 * 3 + 2; i += blockDim.x) red[i] = 0.f;
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(256) void k_head(const float *__restrict__ Z2, cons
       }
       s[k] = acc;
     }
-    const float d = s[1] - s[0];
+    const float d = s[1] - s[0] + (float)pair_offset(recs[idx[p]].depth) / (float)PLAIN_PER_LOGIT;
     const float sig = 1.f / (1.f + expf(-d));
     const float l = d > 0.f ? d + log1pf(expf(-d)) : log1pf(expf(d));
     const float ds[2] = {-sig / (float)P, sig / (float)P};
@@ -313,8 +316,8 @@ int fwd_bwd(const PairRec *recs, int P) {
   if ((r = gemm((const u16 *)g_ms.H1.p, NH1, (const u16 *)g_ms.w2t.p, NH1, (float *)g_ms.Z2.p, NH2, B, NH2, NH1,
                 zr::K_MLP_FWD)))
     return r;
-  ZLAUNCH(zr::K_MLP_BWD, P, k_head, nb(P, 256), 256, 0, (const float *)g_ms.Z2.p, prm, P, (u16 *)g_ms.dZ2.p,
-          (u16 *)g_ms.dZ2t.p, grad, (float *)g_ms.loss.p);
+  ZLAUNCH(zr::K_MLP_BWD, P, k_head, nb(P, 256), 256, 0, (const float *)g_ms.Z2.p, prm, P, recs,
+          (const u32 *)g_ms.idx.p, (u16 *)g_ms.dZ2.p, (u16 *)g_ms.dZ2t.p, grad, (float *)g_ms.loss.p);
   if ((r = gemm((const u16 *)g_ms.dZ2t.p, B, (const u16 *)g_ms.H1t.p, B, grad + I_W2T, NH1, NH2, NH1, B,
                 zr::K_MLP_BWD)))
     return r;
@@ -377,8 +380,13 @@ int ensure(u64 cap) {
   zr::DevBuf nb2;
   if ((r = zr::ensure(nb2, cap * sizeof(PairRec)))) return r;
   if (g_recs.p && n) {
-    const u64 keep = n < cap ? n : cap;
+    // the count runs on past the capacity (the search counts the pairs it
+    // could not store): keep what the old buffer holds, and count that
+    u64 keep = n < g_cap ? n : g_cap;
+    if (keep > cap) keep = cap;
     ZCHECK(hipMemcpyAsync(nb2.p, g_recs.p, keep * sizeof(PairRec), hipMemcpyDeviceToDevice, zr::ctx().stream));
+    ZCHECK(hipMemcpyAsync(g_count.p, &keep, 8, hipMemcpyHostToDevice, zr::ctx().stream));
+    ZCHECK(hipStreamSynchronize(zr::ctx().stream));
   } else {
     ZCHECK(hipMemsetAsync(g_count.p, 0, 8, zr::ctx().stream));
   }

@@ -25,6 +25,7 @@ int fail(int code, const char *fmt, ...) {
 }
 
 int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+  (void)hipGetLastError();  // a failed call leaves the error behind for the next launch check: clear it
   return fail(ZETA_EHIP, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
 }
 

@@ -162,6 +162,7 @@ struct Ply {
   u32 move, best, lastmove;
   u16 mbeg, mcnt;
   u8 stm, hmc, qs, kic, iter, ndefer, busy, searched;
+  u32 ttm;  // the node's TT move (written at its analysis, read by the pair log)
 };
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }  // one wave per block
@@ -445,36 +446,37 @@ __device__ void mlp_score_moves(const SearchArgs &a, const Pos &p, int mbeg, int
   wsync();
 }
 
-// training pairs at a cut node whose cutoff move is quiet (no capture, no
-// promotion, neither the killer nor the counter move) -- the moves the
-// learned ordering ranks: the
-// cutoff move against every quiet move searched before it (scores marked
-// USED, the hard negatives) and against a hash-selected sample of about two
-// quiet moves never searched; wave-compacted
-__device__ __forceinline__ bool plain_quiet(u32 m) {
-  return ((mv_pcpt(m) >> 1) & 7) == PNONE && mv_pto(m) == mv_pfrom(m);
+// the plain ordering's score of a move (src/zeta.cl:2059-2082): MVV-LVA or
+// the piece-square delta, the killer's 1400 and the counter move's 1300
+__device__ __forceinline__ s32 plain_score(u32 stm, u32 m, u32 killer, u32 counter) {
+  s32 sc = order_score(stm, m);
+  sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
+  if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
+    if (m == killer) sc = KILLER_SCORE;
+    else if (counter_code(m) == counter) sc = COUNTER_SCORE;
+  }
+  return sc;
 }
 
+// Training pairs at a cut node of the full-width search whose cutoff move is
+// not the TT move (the TT move goes first whatever the network says): the
+// cutoff move against every other move of the node -- those searched before
+// it, which failed (scores marked USED), and those never searched, which the
+// plain ordering ranked below it.  Each pair carries the plain ordering's
+// score difference (negative minus positive), which the ranking loss adds to
+// the network's, so that the network learns what to add to the plain
+// ordering.  Wave-compacted.
 __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u32 *mmove, const int16_t *mscore,
-                          u32 killer, u32 counter, u32 &nrec) {
-  // counter: the counter move's 16-bit code (counter_code)
-  if (!plain_quiet(n.move) || n.move == killer || counter_code(n.move) == counter) return;
-  const u32 thresh = n.mcnt > 2 ? (u32)(2 * 256 / n.mcnt) : 256u;
+                          u32 ttm, u32 killer, u32 counter, u32 &nrec) {
+  if (n.move == ttm) return;
+  const s32 ppos = plain_score(n.p.stm, n.move, killer, counter);
   for (int c0 = 0; c0 < n.mcnt; c0 += 64) {
     const int k = c0 + lane;
     bool neg = false;
     u32 m = 0;
     if (k < n.mcnt) {
       m = mmove[n.mbeg + k] & ~DEFERRED;
-      if (m != n.move && m != killer && counter_code(m) != counter && plain_quiet(m)) {
-        if (mscore[n.mbeg + k] == (int16_t)USED) {
-          neg = true;
-        } else {
-          u64 z = n.hash ^ ((u64)m * 0x9E3779B97F4A7C15ULL);
-          z = (z ^ (z >> 31)) * 0xBF58476D1CE4E5B9ULL;
-          neg = (u32)((z >> 40) & 0xFF) < thresh;
-        }
-      }
+      neg = m != n.move && m != ttm;
     }
     const u64 mask = __ballot(neg);
     if (!mask) continue;
@@ -485,13 +487,15 @@ __device__ void log_pairs(const SearchArgs &a, const Node &n, int lane, const u3
     if (neg) {
       const u64 slot = base + (u64)popc(mask & ((1ULL << lane) - 1));
       if (slot < a.rec_cap) {
+        s32 dp = plain_score(n.p.stm, m, killer, counter) - ppos;
+        dp = dp > zm::PAIR_OFFSET_MAX ? zm::PAIR_OFFSET_MAX : (dp < -zm::PAIR_OFFSET_MAX ? -zm::PAIR_OFFSET_MAX : dp);
         u64 *rp = a.recs + slot * 6;
         rp[0] = n.p.b0;
         rp[1] = n.p.b1;
         rp[2] = n.p.b2;
         rp[3] = n.p.b3;
         rp[4] = (u64)n.p.stm | ((u64)n.move << 32);
-        rp[5] = (u64)m | ((u64)(u32)n.depth << 32);
+        rp[5] = (u64)m | ((u64)zm::pair_depth_word(n.depth, dp) << 32);
       }
     }
     nrec += cnt;
@@ -957,26 +961,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           if (mlp_r && !n.qs && n.depth >= a.mlp_min_depth && total > 1) {
             wsync();
             mlp_score_moves(a, n.p, n.mbeg, total, lane, hpos, mmove, mscore);
-            // the network orders the quiet moves (no capture, no promotion,
-            // neither killer nor counter move) among themselves, in a band
-            // below every capture, killer and counter move, which keep the
-            // plain scores -- the plain ordering also ranks every capture
-            // (score >= 400) above every quiet move
+            // every move's key is its plain score plus the network's, in
+            // plain-score units (zm::PLAIN_PER_LOGIT per unit of the
+            // network's output, the scale the ranking loss is trained at);
+            // the TT move still goes first
             for (int k = lane; k < total; k += 64) {
               const u32 m = mmove[n.mbeg + k];
-              const bool quiet = ((mv_pcpt(m) >> 1) & 7) == PNONE && mv_pto(m) == mv_pfrom(m);
-              s32 sc;
-              if (quiet && m != kil && counter_code(m) != cm) {
-                sc = (s32)mscore[n.mbeg + k] / 2;
-                sc = -16000 + (sc > 15000 ? 15000 : (sc < -15000 ? -15000 : sc));
-              } else {
-                sc = order_score(n.p.stm, m);
-                sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
-                if (((mv_pcpt(m) >> 1) & 7) == PNONE) {
-                  if (m == kil) sc = KILLER_SCORE;
-                  else if (counter_code(m) == cm) sc = COUNTER_SCORE;
-                }
-              }
+              s32 sc = plain_score(n.p.stm, m, kil, cm) +
+                       (s32)mscore[n.mbeg + k] * zm::PLAIN_PER_LOGIT / 256;
+              sc = sc > 31999 ? 31999 : (sc < -31999 ? -31999 : sc);
               mscore[n.mbeg + k] = (int16_t)sc;
             }
             wsync();
@@ -1057,6 +1050,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
               for (int k = lane; k < total; k += 64)
                 if (mmove[n.mbeg + k] == ttmove) mscore[n.mbeg + k] = (int16_t)TTMOVE_SCORE;
             }
+            if (a.recs && lane == 0) ply[sd].ttm = ttmove;
             if (state == S_NEXT && a.hint && !n.qs) {
               // the hint table (a completed search's table): its move first
               const u32x4 he = *(const u32x4 *)(a.hint + 2 * (key & a.ttmask));
@@ -1195,7 +1189,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           if (a.recs && !n.qs && n.depth >= 1 && n.searched >= 1 && n.mcnt >= 2 &&
               ((u32)(n.hash >> 40) & a.rec_mask) == 0) {
             u32 nrec = 0;
-            log_pairs(a, n, lane, mmove, mscore, killers[sd], (u32)ctab[counter_index(n.lastmove)], nrec);
+            log_pairs(a, n, lane, mmove, mscore, ply[sd].ttm, killers[sd], (u32)ctab[counter_index(n.lastmove)], nrec);
             lcount(cvec, C_RECORDS, nrec);
           }
           // a quiet move that fails high becomes the ply's killer and the
@@ -1224,6 +1218,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       final_score = (rootscore == INF || rootscore == -INF) ? 0 : rootscore;  // src/search.c:137
       final_move = rootmove;
       if (lane == 0 && (!serving || sw_tk(ag_load(&a.sw[r])) == ticket)) atomicMax(&a.depth_done[r], id);
+      // a mate within this iteration's depth is the score of every deeper
+      // iteration too: iterative deepening stops there (src/search.c:177-186)
+      if (is_mate(final_score) && id >= (final_score > 0 ? INF - final_score : INF + final_score)) break;
     }
 
     // ---------------------------------------------------------------- publish

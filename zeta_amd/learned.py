@@ -1,12 +1,17 @@
 """Learned move ordering: record generation, training and evaluation.
 
 Pipeline (all on the GPU through the C-ABI):
-  1. the plain search (MVV-LVA / killer / TT-move ordering, the reference's
-     scheme, src/zeta.cl:2059-2104) runs on seeded playout positions with
-     pair logging on: at every interior cut node the move that caused the
-     cutoff is paired with each move searched before it;
+  1. the plain search (TT move, MVV-LVA / piece-square deltas, killer and
+     counter move: the reference's scheme, src/zeta.cl:2059-2104) runs on
+     seeded playout positions with pair logging on: at sampled cut nodes of
+     the full-width search whose cutoff move is not the TT move, the cutoff
+     move is paired with every other move of the node, each pair carrying
+     the plain ordering's score difference;
   2. the 908-256-32-1 network is trained on those pairs with the pairwise
-     ranking loss (BF16 MFMA forward/backward, Adam);
+     ranking loss softplus(s_neg - s_pos + plain difference / 100) (BF16
+     MFMA forward/backward, Adam): it learns a correction of the plain
+     ordering, which the search applies as plain score + 100 x network
+     output, the TT move still first;
   3. held-out positions are searched with plain and with learned ordering;
      the root scores must be identical (alpha-beta's value does not depend on
      move order) and the node counts / times are compared.
@@ -85,7 +90,7 @@ def train(steps=3000, pairs_per_step=2048, lr=2e-4, seed=1, init_seed=1, held=No
             "seconds": time.perf_counter() - t, "loss_first": first, "heldout_best": best, "curve": curve}
 
 
-def evaluate(seed=40, n=1024, depth=7, workers=1, mlp_min_depth=2, tt_log2=26):
+def evaluate(seed=40, n=1024, depth=7, workers=32, mlp_min_depth=3, tt_log2=26):
     """Plain vs learned ordering on held-out positions; identical root scores required."""
     pos = playouts(seed, n, 20, 60)
     out = {}
@@ -107,13 +112,32 @@ def evaluate(seed=40, n=1024, depth=7, workers=1, mlp_min_depth=2, tt_log2=26):
     return out
 
 
+# parameter layout of the C-ABI: W1[908][256], b1[256], W2[256][32], b2[32], W3[32], b3
+_N_W1, _N_W2 = 908 * 256, 256 * 32
+
+
 def save(path):
-    np.savez_compressed(path, params=mlp_get())
+    """Checkpoint as the search uses it: W1 and W2 as BF16 bit patterns (the
+    search reads them as BF16), the biases and the head as FP32."""
+    p = mlp_get()
+    w1 = p[:_N_W1]
+    w2 = p[_N_W1 + 256:_N_W1 + 256 + _N_W2]
+    rest = np.concatenate([p[_N_W1:_N_W1 + 256], p[_N_W1 + 256 + _N_W2:]])
+
+    def to_bf16(x):
+        u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+        return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    np.savez_compressed(path, w1_bf16=to_bf16(w1), w2_bf16=to_bf16(w2), rest_f32=rest.astype(np.float32))
 
 
 def load(path):
     with np.load(path, allow_pickle=False) as z:
-        p = z["params"]
+        if "params" in z:
+            p = z["params"]
+        else:
+            bf = lambda a: (a.astype(np.uint32) << 16).view(np.float32)  # noqa: E731
+            rest = z["rest_f32"]
+            p = np.concatenate([bf(z["w1_bf16"]), rest[:256], bf(z["w2_bf16"]), rest[256:]])
     if p.size != MLP_NPARAMS:
         raise ValueError("checkpoint has %d parameters, expected %d" % (p.size, MLP_NPARAMS))
     mlp_set(p)
@@ -128,8 +152,8 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--eval-positions", type=int, default=1024)
     ap.add_argument("--eval-depth", type=int, default=7)
-    ap.add_argument("--eval-workers", type=int, default=1)
-    ap.add_argument("--mlp-min-depth", type=int, default=2)
+    ap.add_argument("--eval-workers", type=int, default=32)
+    ap.add_argument("--mlp-min-depth", type=int, default=3)
     ap.add_argument("--checkpoint", default=None)
     ap.add_argument("--load", default=None)
     args = ap.parse_args()
