@@ -40,6 +40,102 @@ HBM_BYTES_PER_POSITION = 12.85
 # k_search: one node per wavefront; SQ_INSTS_VALU / nodes over the depth-5
 # search leg (profiles/search_c3_pmc.md)
 VALU_WAVE_INSTS_PER_NODE = 915.8
+# the ordering network: 908 -> 256 -> 32 -> 1, dense FLOP of one scored move
+# (2 x (908 x 256 + 256 x 32 + 32)); a training pair scores two moves and its
+# step costs about three forward passes (forward, two backward products)
+FLOP_PER_SCORED_MOVE = 2 * (908 * 256 + 256 * 32 + 32)
+MFMA_BF16_PEAK = 2.5e15
+NET_PATH = os.path.join(ROOT, "zeta_amd", "order_net.npz")
+
+
+def log(msg):
+    """Progress of the legs on stderr (the JSON line is the only stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def c4_leg(zeta_amd, depth=7, workers=32, mlp_min_depth=3):
+    """Learned vs plain move ordering: 1024 held-out playout positions (seed
+    40, 20-60 plies), depth 7, one batch each, TT 2^26 cleared before each;
+    then the same search ordered by the best moves the completed search left
+    in the table (order_hint): what a near-perfect ordering would save."""
+    from zeta_amd import learned as L
+    pos = zeta_amd.playouts(40, 1024, 20, 60)
+    out = {"workload": "1024 playout positions seed 40, 20-60 plies, depth %d + quiescence, TT 2^26 x 16 B, "
+                       "%d workers max per root" % (depth, workers), "mlp_min_depth": mlp_min_depth,
+           "network": os.path.relpath(NET_PATH, ROOT)}
+    ref = None
+    for mode in ("plain", "learned", "hinted"):
+        if mode == "learned":
+            L.load(NET_PATH)
+        t = time.perf_counter()
+        r = zeta_amd.search(pos, depth, workers=workers, tt_log2=26, clear_tt=True, use_mlp=(mode == "learned"),
+                            mlp_min_depth=mlp_min_depth, order_hint=(mode == "hinted"))
+        dt = time.perf_counter() - t
+        ctr = zeta_amd.search_counters()
+        nodes = int(r["nodes"].sum())
+        out[mode] = {"seconds": round(dt, 3), "nodes": nodes, "nodes_per_s": nodes / dt,
+                     "completed": int((r["status"] == 0).sum()), "mlp_nodes": ctr["mlp_nodes"]}
+        if ref is None:
+            ref = r["score"].copy()
+            out["score_checksum"] = int(ref.astype(np.int64).sum())
+        else:
+            out[mode]["identical_scores"] = bool((r["score"] == ref).all())
+            out[mode]["node_reduction"] = round(1 - nodes / out["plain"]["nodes"], 4)
+            out[mode]["time_reduction"] = round(1 - dt / out["plain"]["seconds"], 4)
+        log("C4 %s: %.1f s, %d nodes" % (mode, dt, nodes))
+    out["learned_vs_plain"] = {"nodes": [out["plain"]["nodes"], out["learned"]["nodes"]],
+                               "ms": [round(out["plain"]["seconds"] * 1e3), round(out["learned"]["seconds"] * 1e3)],
+                               "identical_scores": out["learned"]["identical_scores"]}
+    return out
+
+
+def mlp_leg(zeta_amd, steps=400, pairs_per_step=2048):
+    """Training throughput of the ordering network's BF16 MFMA kernels on pairs
+    logged by a small plain search (256 playout positions, seed 4, depth 4)."""
+    zeta_amd.records_reserve(600_000)
+    zeta_amd.records_clear()
+    pos = zeta_amd.playouts(4, 256, 20, 60)
+    zeta_amd.search(pos, 4, workers=4, tt_log2=22, clear_tt=True, log_records=True, log_shift=0)
+    npairs = zeta_amd.records_count()
+    zeta_amd.mlp_init(1)
+    zeta_amd.mlp_train(20, pairs_per_step, 2e-4, 7)
+    zeta_amd.synchronize()
+    zeta_amd.stats_reset()
+    zeta_amd.set_timing(True)
+    t = time.perf_counter()
+    zeta_amd.mlp_train(steps, pairs_per_step, 2e-4, 11)
+    zeta_amd.synchronize()
+    dt = time.perf_counter() - t
+    kst = zeta_amd.stats()
+    zeta_amd.set_timing(False)
+    zeta_amd.stats_reset()
+    zeta_amd.records_clear()
+    flop = 3.0 * 2 * pairs_per_step * FLOP_PER_SCORED_MOVE * steps
+    kms = {k: round(v[1], 3) for k, v in kst.items() if v[0] and k.startswith("mlp")}
+    gemm_s = (kms.get("mlp_fwd", 0) + kms.get("mlp_bwd", 0)) / 1e3
+    log("MLP training: %d steps in %.2f s" % (steps, dt))
+    return {"workload": "%d Adam steps of %d pairs (logged by a depth-4 search of 256 seed-4 positions, %d pairs)"
+                        % (steps, pairs_per_step, npairs),
+            "seconds": round(dt, 3), "pairs_per_s": steps * pairs_per_step / dt, "kernels_ms_total": kms,
+            "roofline": {"bound": "mfma", "achieved": round(flop / max(gemm_s, 1e-9) / 1e12, 3),
+                         "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                         "frac": round(flop / max(gemm_s, 1e-9) / MFMA_BF16_PEAK, 5),
+                         "flop_per_step": 3 * 2 * pairs_per_step * FLOP_PER_SCORED_MOVE,
+                         "note": "dense FLOP over the summed time of the forward and backward kernels"}}
+
+
+def c5_leg(zeta_amd, rate=2000.0, n=48, max_seconds=60.0, workers=32):
+    """Serving: a Poisson stream (seed 5) of n requests at `rate`, depths
+    uniform in 4-9, plain then learned ordering, each stream capped at
+    max_seconds (requests still running then are cancelled and reported)."""
+    from zeta_amd import learned as L
+    from zeta_amd import serving as S
+    L.load(NET_PATH)
+    rep = S.compare(rate, n, seed=5, dmin=4, dmax=9, slots=256, workers=workers, max_seconds=max_seconds,
+                    mlp_min_depth=3, progress=lambda d: log("C5 " + json.dumps(d)))
+    for mode in ("plain", "learned"):
+        rep[mode].pop("by_depth", None)
+    return rep
 
 
 def host_info():
@@ -92,6 +188,9 @@ def main():
     ap.add_argument("--no-search", action="store_true", help="skip the fixed-depth search leg")
     ap.add_argument("--search-depth", type=int, default=6)
     ap.add_argument("--search-workers", type=int, default=32)
+    ap.add_argument("--no-c4", action="store_true", help="skip the learned-vs-plain ordering leg (depth 7)")
+    ap.add_argument("--no-c5", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the ordering network's training leg")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
                     help="also run the library's streaming copy kernel once over BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -221,6 +320,12 @@ def main():
                                "valu_wave_insts_per_node": VALU_WAVE_INSTS_PER_NODE},
                   **ctr}
 
+    if search is not None:
+        log("C3 search: %.1f s" % search["time_to_depth_s"])
+    c4 = None if args.no_c4 else c4_leg(zeta_amd)
+    mlp = None if args.no_mlp else mlp_leg(zeta_amd)
+    c5 = None if args.no_c5 else c5_leg(zeta_amd)
+
     cpu = None
     c1_cpu = None
     if not args.no_cpu_baseline:
@@ -241,16 +346,20 @@ def main():
         c1_cpu = {"value": cc / best, "unit": "leaves/s", "cores": 1, "kind": "port", "leaves": cc,
                   "sample": "oracle startpos perft(5) on one thread, best of 3 (%.3f s)" % best}
         if search is not None:
-            # the oracle's alpha-beta (same node definition: moves made) on a
-            # bounded sample of the same positions
+            # the oracle's alpha-beta (same node definition: moves made, the
+            # same iterative deepening 1..depth) on a bounded sample of the
+            # same positions at the same depth
             from oracle import refcpu
             t3 = time.perf_counter()
-            _, cn = refcpu.search_batch(spos[:256], 5, threads)
+            cs, cn = refcpu.search_id_batch(spos[:128], args.search_depth, threads)
             dt3 = time.perf_counter() - t3
             search["cpu_baseline"] = {"value": float(cn.sum()) / dt3, "unit": "nodes/s", "cores": threads,
                                       "kind": "port",
-                                      "sample": "oracle search of the first 256 positions at depth 5 "
-                                                "(%d nodes, %.1f s, %d threads)" % (int(cn.sum()), dt3, threads)}
+                                      "sample": "oracle iterative deepening 1..%d of the first 128 positions "
+                                                "(%d nodes, %.1f s, %d threads; no TT)"
+                                                % (args.search_depth, int(cn.sum()), dt3, threads),
+                                      "scores_equal_device": bool((cs == res["score"][:128]).all())}
+        log("CPU baselines done")
 
     out = {"metric": "perft nodes/s", "value": value, "unit": "leaves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -263,7 +372,7 @@ def main():
            "roofline": roof, "cpu_baseline": cpu, "nobulk": nobulk,
            "config1": {"workload": "startpos perft(5), one root", "leaves": c1, "ms": round(c1_ms, 3),
                        "leaves_per_s": c1 / (c1_ms / 1e3), "cpu_baseline": c1_cpu},
-           "search": search}
+           "search": search, "learned_ordering": c4, "mlp_training": mlp, "serving": c5}
     if calib:
         out["calibration"] = calib
     print(json.dumps(out))

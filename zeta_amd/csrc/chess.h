@@ -303,15 +303,17 @@ ZD void analyse(const Pos &p, Legal &L) {
   L.cmask = n == 0 ? FULL : (n == 1 ? (checkers | chkray) : 0ULL);
 }
 
-// is the side to move's king attacked after the e.p. capture from->to taking sqep?
+// is the side to move's king attacked after the e.p. capture from->to taking
+// sqep?  A knight or pawn checks it after the capture exactly when it checks
+// it before (unless it is the captured pawn); sliders by the line attacks of
+// the king's square on the new occupancy (two single-square slider lookups,
+// not eight occluded fills); a king never checks
 ZD bool ep_legal(const Legal &L, int from, int to, int sqep) {
+  if (!L.kbb) return true;
   const u64 occ2 = (L.occ ^ bit(from) ^ bit(sqep)) | bit(to);
   const u64 opp2 = L.opp & ~bit(sqep);
-  const u64 e = ~occ2;
-  u64 att = (rook_att(L.kbb, e) & opp2 & L.rq) | (bishop_att(L.kbb, e) & opp2 & L.bq);
-  att |= knight_att(L.kbb) & opp2 & L.knights;
-  att |= pawn_att(L.kbb, L.stm) & opp2 & L.pawns;
-  att |= king_att(L.kbb) & opp2 & L.kings;
+  const u64 att = (rook_sq(L.ksq, occ2) & opp2 & L.rq) | (bishop_sq(L.ksq, occ2) & opp2 & L.bq) |
+                  (L.checkers & ~(L.rq | L.bq) & ~bit(sqep));
   return att == 0;
 }
 

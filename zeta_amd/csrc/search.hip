@@ -534,6 +534,33 @@ __device__ __forceinline__ int sw_max(u64 v) { return (int)((v >> 16) & 0xFF); }
 __device__ __forceinline__ int sw_w(u64 v) { return (int)((v >> 2) & 0x3FFF); }
 __device__ __forceinline__ int sw_st(u64 v) { return (int)(v & 3); }
 
+// Serving: a slot's node count and deepest iteration carry the low 16 bits of
+// their request's ticket (count: tag << 48 | nodes; depth: tag << 16 |
+// iteration), and a worker adds to them only while the tag is its request's,
+// in one compare-and-swap: a late helper of an earlier request can neither
+// write into the slot's next request nor see its own numbers mixed with it.
+__device__ __forceinline__ u64 slot_tag(u64 tk) { return tk & 0xFFFFULL; }
+__device__ __forceinline__ void slot_add_nodes(u64 *p, u64 tk, u64 add) {
+  u64 v = ag_load(p);
+  while ((v >> 48) == slot_tag(tk)) {
+    const u64 nv = (v & ~0xFFFFFFFFFFFFULL) | (((v & 0xFFFFFFFFFFFFULL) + add) & 0xFFFFFFFFFFFFULL);
+    u64 e = v;
+    if (__hip_atomic_compare_exchange_strong(p, &e, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
+    v = e;
+  }
+}
+__device__ __forceinline__ void slot_max_depth(s32 *p, u64 tk, s32 id) {
+  s32 v = ag_load(p);
+  while ((u64)((u32)v >> 16) == slot_tag(tk) && (v & 0xFFFF) < id) {
+    s32 e = v;
+    const s32 nv = (s32)((u32)(slot_tag(tk) << 16) | (u32)id);
+    if (__hip_atomic_compare_exchange_strong(p, &e, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return;
+    v = e;
+  }
+}
+
 __device__ __forceinline__ u32 xcc_id() {
   // HW_REG_XCC_ID (hwreg 20), bits [3:0]
   return (u32)__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 15u;
@@ -609,8 +636,8 @@ __device__ int serve_pick(const SearchArgs &a, int lane, u64 &tk_out) {
             ag_store(&d.flags, (s32)sys_load32((const u32 *)&h.flags));
             ag_store(&d.workers, hw);
             sys_store(&a.sack[qi], c + 1);
-            ag_store(&a.nodes[s], (u64)0);
-            ag_store(&a.depth_done[s], 0);
+            ag_store(&a.nodes[s], slot_tag(c + 1) << 48);
+            ag_store(&a.depth_done[s], (s32)(u32)(slot_tag(c + 1) << 16));
             const u64 wmax = (u64)(hw < 1 ? 1 : (hw > 255 ? 255 : hw));
             // release: the request copy above is visible to every helper
             // that joins through this word (one write-back per request)
@@ -694,9 +721,9 @@ __device__ void serve_leave(const SearchArgs &a, int r, u64 tk, bool aborted, bo
       ServeSlot &sl = a.slots[r];
       sys_store(&sl.score, pub == SV_DONE ? score : 0);
       sys_store(&sl.bestmove, pub == SV_DONE ? move : 0u);
-      sys_store(&sl.nodes, ag_load(&a.nodes[r]));
+      sys_store(&sl.nodes, (u64)(ag_load(&a.nodes[r]) & 0xFFFFFFFFFFFFULL));
       sys_store(&sl.ticket, tk);
-      sys_store(&sl.depth_done, ag_load(&a.depth_done[r]));
+      sys_store(&sl.depth_done, (s32)(ag_load(&a.depth_done[r]) & 0xFFFF));
       sys_store(&sl.status, (s32)pub);
       // the one ordering point: the result fields before the done flag
       __hip_atomic_store(&sl.state, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -770,7 +797,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     // iteration in progress: the earlier ones would only repeat work whose
     // moves are in the shared TT (the final iteration alone sets the score)
     int id0 = 1;
-    if (lane == 0) id0 = __atomic_load_n(&a.depth_done[r], __ATOMIC_RELAXED) + 1;
+    if (lane == 0) id0 = (__atomic_load_n(&a.depth_done[r], __ATOMIC_RELAXED) & (serving ? 0xFFFF : ~0)) + 1;
     id0 = uf(id0);
     id0 = id0 < 1 ? 1 : (id0 > depth_r ? depth_r : id0);
     for (int id = id0; id <= depth_r && !aborted && !failed; id++) {
@@ -1217,7 +1244,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       }
       final_score = (rootscore == INF || rootscore == -INF) ? 0 : rootscore;  // src/search.c:137
       final_move = rootmove;
-      if (lane == 0 && (!serving || sw_tk(ag_load(&a.sw[r])) == ticket)) atomicMax(&a.depth_done[r], id);
+      if (lane == 0) {
+        if (serving) slot_max_depth(&a.depth_done[r], ticket, id);
+        else atomicMax(&a.depth_done[r], id);
+      }
       // a mate within this iteration's depth is the score of every deeper
       // iteration too: iterative deepening stops there (src/search.c:177-186)
       if (is_mate(final_score) && id >= (final_score > 0 ? INF - final_score : INF + final_score)) break;
@@ -1226,8 +1256,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     // ---------------------------------------------------------------- publish
     if (lane < C_NODES && cvec) atomicAdd((unsigned long long *)&a.ctr[lane], (unsigned long long)cvec);
     if (lane == 0) {
-      if (!serving || sw_tk(ag_load(&a.sw[r])) == ticket)
-        atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
+      // a served request's count holds the nodes of the workers that left it
+      // before its result was published; the session total holds them all
+      if (serving) slot_add_nodes(&a.nodes[r], ticket, nodes);
+      else atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
       atomicAdd((unsigned long long *)&a.ctr[C_NODES], (unsigned long long)nodes);
       if (serving) {
         serve_leave(a, r, ticket, aborted, failed, final_score, final_move);

@@ -64,6 +64,12 @@ def train(steps=3000, pairs_per_step=2048, lr=2e-4, seed=1, init_seed=1, held=No
     `patience` evaluations without improvement and keeps the best weights
     (the network overfits the positions it was trained on quickly)."""
     mlp_init(init_seed)
+    # the output layer starts at zero: the untrained network adds nothing to
+    # the plain ordering, and training (with early stopping on held-out
+    # pairs) moves away from it only where that ranks held-out pairs better
+    p = mlp_get()
+    p[_N_W1 + 256 + _N_W2 + 32:] = 0.0
+    mlp_set(p)
     t = time.perf_counter()
     if held is None:
         losses = mlp_train(steps, pairs_per_step, lr, seed)
