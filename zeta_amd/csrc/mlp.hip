@@ -51,20 +51,26 @@ struct PairRec {  // one logged training pair (48 B)
 
 constexpr u16 BF_ONE = 0x3F80;
 
-// ---- NT GEMM on MFMA: one wavefront per 32x32 tile of C
+// ---- NT GEMM on MFMA: one wavefront per 32x32 tile of C and slice of K.
+// The weight-gradient products have few output tiles and a long K (the
+// batch): their K is split over ksplit wavefronts per tile, which add their
+// partial tiles into C (zeroed before) with FP32 atomics.
 __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int lda, const u16 *__restrict__ Bt,
-                                                 int ldb, float *__restrict__ C, int ldc, int M, int N, int K) {
+                                                 int ldb, float *__restrict__ C, int ldc, int M, int N, int K,
+                                                 int ksplit) {
   const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int tn_count = N / 32;
-  const int tm = wave / tn_count, tn = wave % tn_count;
+  const int tile = wave / ksplit, ks = wave % ksplit;
+  const int tm = tile / tn_count, tn = tile % tn_count;
   if (tm * 32 >= M) return;
+  const int kc = K / ksplit, k0 = ks * kc;
   const int r = tm * 32 + (lane & 31), c = tn * 32 + (lane & 31);
   const int kofs = (lane >> 5) * 8;
-  const u16 *pa = A + (size_t)r * lda + kofs;
-  const u16 *pb = Bt + (size_t)c * ldb + kofs;
+  const u16 *pa = A + (size_t)r * lda + kofs + k0;
+  const u16 *pb = Bt + (size_t)c * ldb + kofs + k0;
   f32x16 acc = {};
-  for (int k = 0; k < K; k += 16) {
+  for (int k = 0; k < kc; k += 16) {
     const bf16x8 a = *(const bf16x8 *)(pa + k);
     const bf16x8 b = *(const bf16x8 *)(pb + k);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
@@ -73,7 +79,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int row = tm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-    C[(size_t)row * ldc + tn * 32 + (lane & 31)] = acc[i];
+    float *dst = C + (size_t)row * ldc + tn * 32 + (lane & 31);
+    if (ksplit == 1) *dst = acc[i];
+    else atomicAdd(dst, acc[i]);
   }
 }
 
@@ -117,21 +125,32 @@ __global__ void k_l1_act(float *Z1, const float *__restrict__ b1, u16 *H1, u16 *
 }
 
 // layer 2 activation, layer 3, pairwise ranking loss and the gradients that
-// need no GEMM; one thread per pair (rows p and P+p)
+// need no GEMM; one thread per pair (rows p and P+p).  The sums over pairs
+// (b2, W3, b3 gradients, the loss) are wave reductions, then one LDS slot per
+// wave and one global atomic per block and value.
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_head(const float *__restrict__ Z2, const float *__restrict__ prm, int P,
                                               const PairRec *__restrict__ recs, const u32 *__restrict__ idx,
                                               u16 *dZ2, u16 *dZ2t, float *grad, float *loss_out) {
-  __shared__ float red[NH2 * 2 + NH2 + 2];
-  for (int i = threadIdx.x; i < NH2 * 3 + 2; i += blockDim.x) red[i] = 0.f;
-  __syncthreads();
+  constexpr int NV = NH2 * 2 + 2;  // b2, W3, b3, loss
+  __shared__ float red[4][NV];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int B = 2 * P;
   const float *b2 = prm + I_B2, *w3 = prm + I_W3;
+  float gb2[NH2], gw3[NH2], gb3 = 0.f, gl = 0.f;
+#pragma unroll
+  for (int o = 0; o < NH2; o++) gb2[o] = gw3[o] = 0.f;
   if (p < P) {
     float h[2][NH2], s[2];
     for (int k = 0; k < 2; k++) {
       const int e = p + k * P;
       float acc = prm[I_B3];
+#pragma unroll
       for (int o = 0; o < NH2; o++) {
         float z = Z2[(size_t)e * NH2 + o] + b2[o];
         z = z > 0.f ? z : 0.f;
@@ -146,26 +165,43 @@ __global__ __launch_bounds__(256) void k_head(const float *__restrict__ Z2, cons
     const float ds[2] = {-sig / (float)P, sig / (float)P};
     for (int k = 0; k < 2; k++) {
       const int e = p + k * P;
+#pragma unroll
       for (int o = 0; o < NH2; o++) {
         const float dz = h[k][o] > 0.f ? ds[k] * w3[o] : 0.f;
         const u16 db = f2bf(dz);
         dZ2[(size_t)e * NH2 + o] = db;
         dZ2t[(size_t)o * B + e] = db;
-        atomicAdd(&red[o], dz);                 // b2
-        atomicAdd(&red[NH2 + o], ds[k] * h[k][o]);  // w3
+        gb2[o] += dz;
+        gw3[o] += ds[k] * h[k][o];
       }
-      atomicAdd(&red[NH2 * 2], ds[k]);  // b3
+      gb3 += ds[k];
     }
-    atomicAdd(&red[NH2 * 2 + 1], l / (float)P);
+    gl = l / (float)P;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 0; o < NH2; o++) {
+    const float a = wave_sum_f(gb2[o]), b = wave_sum_f(gw3[o]);
+    if (lane == 0) {
+      red[w][o] = a;
+      red[w][NH2 + o] = b;
+    }
+  }
+  gb3 = wave_sum_f(gb3);
+  gl = wave_sum_f(gl);
+  if (lane == 0) {
+    red[w][2 * NH2] = gb3;
+    red[w][2 * NH2 + 1] = gl;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < NH2; i += blockDim.x) {
-    atomicAdd(&grad[I_B2 + i], red[i]);
-    atomicAdd(&grad[I_W3 + i], red[NH2 + i]);
-  }
-  if (threadIdx.x == 0) {
-    atomicAdd(&grad[I_B3], red[NH2 * 2]);
-    atomicAdd(loss_out, red[NH2 * 2 + 1]);
+  const int nw = (int)(blockDim.x >> 6);
+  for (int i = threadIdx.x; i < NV; i += blockDim.x) {
+    float t = 0.f;
+    for (int k = 0; k < nw; k++) t += red[k][i];
+    if (i < NH2) atomicAdd(&grad[I_B2 + i], t);
+    else if (i < 2 * NH2) atomicAdd(&grad[I_W3 + i - NH2], t);
+    else if (i == 2 * NH2) atomicAdd(&grad[I_B3], t);
+    else atomicAdd(loss_out, t);
   }
 }
 
@@ -288,9 +324,15 @@ int ensure_batch(int P) {
 
 inline unsigned nb(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, float *C, int ldc, int M, int N, int K, int kid) {
-  const size_t waves = (size_t)(M / 32) * (N / 32);
-  ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt, nb(waves * 64, 256), 256, 0, A, lda, Bt, ldb, C, ldc, M, N, K);
+// C = A . Bt^T.  accumulate = 1: C was zeroed and the K slices may add into
+// it, so a product with few output tiles splits its K to fill the device
+int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, float *C, int ldc, int M, int N, int K, int kid,
+         int accumulate = 0) {
+  const size_t tiles = (size_t)(M / 32) * (N / 32);
+  int ksplit = 1;
+  while (accumulate && tiles * ksplit < 2048 && K % (32 * ksplit) == 0 && K / (2 * ksplit) >= 128) ksplit *= 2;
+  ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt, nb(tiles * ksplit * 64, 256), 256, 0, A, lda, Bt, ldb, C, ldc, M, N,
+          K, ksplit);
   return ZETA_OK;
 }
 
@@ -319,7 +361,7 @@ int fwd_bwd(const PairRec *recs, int P) {
   ZLAUNCH(zr::K_MLP_BWD, P, k_head, nb(P, 256), 256, 0, (const float *)g_ms.Z2.p, prm, P, recs,
           (const u32 *)g_ms.idx.p, (u16 *)g_ms.dZ2.p, (u16 *)g_ms.dZ2t.p, grad, (float *)g_ms.loss.p);
   if ((r = gemm((const u16 *)g_ms.dZ2t.p, B, (const u16 *)g_ms.H1t.p, B, grad + I_W2T, NH1, NH2, NH1, B,
-                zr::K_MLP_BWD)))
+                zr::K_MLP_BWD, 1)))
     return r;
   if ((r = gemm((const u16 *)g_ms.dZ2.p, NH2, (const u16 *)g_ms.w2.p, NH2, (float *)g_ms.dH1.p, NH1, B, NH1, NH2,
                 zr::K_MLP_BWD)))
@@ -327,7 +369,7 @@ int fwd_bwd(const PairRec *recs, int P) {
   ZLAUNCH(zr::K_MLP_BWD, B, k_l1_back, NH1, 256, 0, (const float *)g_ms.dH1.p, (const float *)g_ms.Z1.p, B,
           (u16 *)g_ms.dZ1t.p, grad);
   if ((r = gemm((const u16 *)g_ms.dZ1t.p, B, (const u16 *)g_ms.Xt.p, B, grad + I_W1T, NINP, NH1, NINP, B,
-                zr::K_MLP_BWD)))
+                zr::K_MLP_BWD, 1)))
     return r;
   return ZETA_OK;
 }

@@ -145,6 +145,7 @@ struct SearchArgs {
   ServeDev *sv;       // device
   u64 *sw;            // device slot words
   ServeReq *dreq;     // device copies of the claimed requests
+  u64 *dres;          // per slot: score | best move << 32 of the worker that completed the request
   u64 *dcancel;       // device mirror of scancel
   u64 watchdog_ticks; // wall-clock ticks without a heartbeat before the grid exits (0 = never)
 };
@@ -702,25 +703,36 @@ __device__ __forceinline__ bool serve_abort(const SearchArgs &a, int r, u64 tk) 
   return sw_tk(v) != tk || sw_st(v) != SV_RUN || ag_load(&a.dcancel[r]) == tk || ag_load(&a.sv->shut) != 0;
 }
 
-// serving: leave the request; the worker that completes it (or the last one
-// to leave an aborted request) writes the result to the host slot
+// serving: leave the request.  The worker that completes it (or fails it)
+// records the result in device memory and sets the status in the slot word,
+// which sends the others away at their next poll; the last worker to leave
+// publishes the result to the host slot -- DONE or FAILED as set, or
+// CANCELLED when every worker left a running request -- so the published
+// node count includes every worker's nodes.
 __device__ void serve_leave(const SearchArgs &a, int r, u64 tk, bool aborted, bool failed, s32 score, u32 move) {
+  if (!aborted && !failed) ag_store(&a.dres[r], (u64)(u32)score | ((u64)move << 32));
   for (;;) {
     const u64 v = ag_load(&a.sw[r]);
     if (sw_tk(v) != tk) return;  // the slot has moved on
-    u64 nv = v - 4;
-    int pub = -1;
-    if (sw_st(v) == SV_RUN) {
-      if (failed) pub = SV_FAILED;
-      else if (!aborted) pub = SV_DONE;
-      else if (sw_w(v) == 1) pub = SV_CANCELLED;
-      if (pub >= 0) nv = (nv & ~3ULL) | (u64)pub;
+    int st = sw_st(v);
+    if (st == SV_RUN) {
+      if (failed) st = SV_FAILED;
+      else if (!aborted) st = SV_DONE;
     }
-    if (!ag_cas(&a.sw[r], v, nv)) continue;
-    if (pub >= 0) {
+    const bool last = sw_w(v) == 1;
+    const u64 nv = ((v - 4) & ~3ULL) | (u64)st;
+    // release: the result recorded above before the status that announces
+    // it; acquire: the last worker sees the result the completing one wrote
+    u64 e = v;
+    if (!__hip_atomic_compare_exchange_strong(&a.sw[r], &e, nv, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+      continue;
+    if (last) {
+      const int pub = st == SV_RUN ? SV_CANCELLED : st;
+      const u64 res = pub == SV_DONE ? ag_load(&a.dres[r]) : 0ULL;
       ServeSlot &sl = a.slots[r];
-      sys_store(&sl.score, pub == SV_DONE ? score : 0);
-      sys_store(&sl.bestmove, pub == SV_DONE ? move : 0u);
+      sys_store(&sl.score, (s32)(u32)res);
+      sys_store(&sl.bestmove, (u32)(res >> 32));
       sys_store(&sl.nodes, (u64)(ag_load(&a.nodes[r]) & 0xFFFFFFFFFFFFULL));
       sys_store(&sl.ticket, tk);
       sys_store(&sl.depth_done, (s32)(ag_load(&a.depth_done[r]) & 0xFFFF));
@@ -1256,8 +1268,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     // ---------------------------------------------------------------- publish
     if (lane < C_NODES && cvec) atomicAdd((unsigned long long *)&a.ctr[lane], (unsigned long long)cvec);
     if (lane == 0) {
-      // a served request's count holds the nodes of the workers that left it
-      // before its result was published; the session total holds them all
+      // before leaving: the last worker to leave publishes the request's
+      // count, which then holds every worker's nodes
       if (serving) slot_add_nodes(&a.nodes[r], ticket, nodes);
       else atomicAdd((unsigned long long *)&a.nodes[r], (unsigned long long)nodes);
       atomicAdd((unsigned long long *)&a.ctr[C_NODES], (unsigned long long)nodes);
@@ -1600,7 +1612,7 @@ struct ServeState {
   ServeHost *sh = nullptr;
   u64 *queue = nullptr, *ack = nullptr, *cancel = nullptr;
   ServeSlot *slots = nullptr;
-  zr::DevBuf sv, sw, dreq, dcancel, nodes, depth_done, busy, ctr, err, rbound;
+  zr::DevBuf sv, sw, dreq, dcancel, nodes, depth_done, busy, ctr, err, rbound, dres;
   std::vector<u64> slot_ticket;  // ticket + 1 held by the slot, 0 = free
   std::vector<u64> slot_cost;
   std::vector<int> free_slots;
@@ -1657,7 +1669,7 @@ void sv_take(int s, zeta_result *res) {
 void sv_free_all() {
   zr::release(g_sv.sv); zr::release(g_sv.sw); zr::release(g_sv.dreq); zr::release(g_sv.dcancel);
   zr::release(g_sv.nodes); zr::release(g_sv.depth_done); zr::release(g_sv.busy); zr::release(g_sv.ctr);
-  zr::release(g_sv.err); zr::release(g_sv.rbound);
+  zr::release(g_sv.err); zr::release(g_sv.rbound); zr::release(g_sv.dres);
   if (g_sv.pinned) (void)hipHostFree(g_sv.pinned);
   g_sv.pinned = nullptr;
   g_sv.sh = nullptr;
@@ -1723,7 +1735,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
       (r = zr::ensure(S.dreq, (size_t)S.nslots * sizeof(ServeReq))) || (r = zr::ensure(S.dcancel, (size_t)S.nslots * 8)) ||
       (r = zr::ensure(S.nodes, (size_t)S.nslots * 8)) || (r = zr::ensure(S.depth_done, (size_t)S.nslots * 4)) ||
       (r = zr::ensure(S.busy, (size_t)busy_entries * 4)) || (r = zr::ensure(S.ctr, C_NCTR * 8)) || (r = zr::ensure(S.err, 64)) ||
-      (r = zr::ensure(S.rbound, (size_t)S.nslots * 8))) {
+      (r = zr::ensure(S.rbound, (size_t)S.nslots * 8)) || (r = zr::ensure(S.dres, (size_t)S.nslots * 8))) {
     sv_free_all();
     return r;
   }
@@ -1781,6 +1793,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   sa.ctr = (u64 *)S.ctr.p;
   sa.habort = zr::ctx().abort_dev;
   sa.rbound = (u64 *)S.rbound.p;
+  sa.dres = (u64 *)S.dres.p;
   sa.use_mlp = S.use_mlp ? 1 : 0;
   sa.mlp_min_depth = o->mlp_min_depth;
   if (w) sa.w = *w;
