@@ -52,6 +52,13 @@ extern uint64_t *GLOBAL_HASHHISTORY;
 extern uint64_t *COUNTERS;
 extern uint32_t *PV;
 
+/* The shim's own launch knobs (no counterpart in the reference; defined by
+ * the shim, 0 = library default, so a host written against src/clrun.h
+ * needs nothing more): zeta_search_opts.split_depth / .waves_per_simd, which
+ * the front end reads from config.txt keys of the same names. */
+extern int32_t zeta_split_depth;
+extern int32_t zeta_waves_per_simd;
+
 /* src/clrun.h:26 -- binds the device (zeta_init) once; later calls are no-ops */
 bool cl_init_device(char *kernelname);
 /* src/clrun.h:27 -- snapshots GLOBAL_BOARD and the hash history for the next run */

@@ -134,6 +134,18 @@ typedef struct {
    * a near-minimal tree: its node count measures what any move ordering
    * could save.  Scores are unchanged (any order gives them). */
   int order_hint;
+  /* Launch and split knobs (set by the front end's autotuner, config.txt
+   * keys split_depth / waves_per_simd; 0 = default):
+   * split_depth: ABDADA -- a worker defers a sibling that another worker is
+   *   searching, and takes a busy-table slot for the one it searches, at
+   *   nodes with at least this much depth left below the child (default 2);
+   *   the reference gates ABDADA on any full-width node below the root
+   *   (src/zeta.cl:1648-1660, 1743-1750), here the depth is a knob.
+   * waves_per_simd: persistent search wavefronts per SIMD, 1..2 (default:
+   *   as many as fit, 2 on gfx950); replaces the work-group count the
+   *   reference tunes through threadsY (src/clconfig.c:878-960). */
+  int split_depth;
+  int waves_per_simd;
 } zeta_search_opts;
 #define ZETA_HISTORY 128
 

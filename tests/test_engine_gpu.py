@@ -3,6 +3,7 @@ reference's six-function device boundary behind it (include/zeta_clrun.h):
 the reference's own command flows -- selftest, perft at the `sd` depth, an
 xboard game -- run unchanged over libzetahip.so."""
 import os
+import re
 import subprocess
 
 import pytest
@@ -30,6 +31,25 @@ def test_selftest_passes_all_reference_kats():
     # the reference's summary banner (src/test.c:144-146)
     assert "###############################\n### passed 14 from 14 tests ###\n###############################" \
         in p.stdout
+
+
+def test_guessconfig_writes_and_reads_back_the_launch_knobs(tmp_path):
+    # the autotuner (--guessconfig, src/clconfig.c:878-960): workers per root,
+    # ABDADA split depth and wavefronts per SIMD measured on the GPU, written
+    # to config.txt with the reference's keys plus this build's two, and read
+    # back; then the engine starts from that file and searches with it
+    cfg = str(tmp_path / "config.txt")
+    p = run("", args=("--guessconfig",), config=cfg, timeout=200)
+    assert p.returncode == 0
+    vals = {k: int(v) for k, v in re.findall(r"^(\w+): (-?\d+);", open(cfg).read(), re.M)}
+    assert vals["threadsX"] in (1, 4, 16, 64) and vals["threadsY"] == 1
+    assert vals["split_depth"] in (1, 2, 3) and vals["waves_per_simd"] in (1, 2)
+    assert vals["nodes_per_second"] > 0 and vals["tt1_memory"] == 1024
+    assert ("#> config read back: threadsX %d, split_depth %d, waves_per_simd %d"
+            % (vals["threadsX"], vals["split_depth"], vals["waves_per_simd"])) in p.stdout
+    q = run("new\nsd 4\ngo\nquit\n", config=cfg)
+    assert q.returncode == 0
+    assert re.search(r"^move [a-h][1-8][a-h][1-8]", q.stdout, re.M)
 
 
 def test_perft_command_startpos_depth5():

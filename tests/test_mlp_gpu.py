@@ -39,8 +39,11 @@ def test_params_roundtrip(zeta):
     assert np.array_equal(zeta.mlp_get(), w)
 
 
-def test_gradient_matches_oracle(zeta, oracle):
-    pairs = _pairs(oracle, 64, 1)
+@pytest.mark.parametrize("npairs", [64, 2048])
+def test_gradient_matches_oracle(zeta, oracle, npairs):
+    # 64 pairs: each K-split product in one block per tile; 2048 pairs (the
+    # training batch): several blocks per tile adding into the gradient
+    pairs = _pairs(oracle, npairs, 1)
     w = np.random.default_rng(5).normal(0, 0.08, M.NPARAMS).astype(np.float32)
     zeta.mlp_set(w)
     loss, g = zeta.mlp_grad(pairs)

@@ -164,6 +164,18 @@ def test_workers_and_tt_do_not_change_scores(zeta, oracle):
             assert (got["score"] == want).all(), (workers, tt)
 
 
+def test_launch_knobs_do_not_change_scores(zeta, oracle):
+    # the autotuner's knobs: ABDADA split depth and wavefronts per SIMD
+    # (64 roots x 64 workers ask for more wavefronts than one per SIMD)
+    pos = oracle.gen_playouts(3, 64, 20, 60)
+    want, _ = oracle.search_batch(pos, 4)
+    for split_depth, waves in ((1, 1), (3, 2), (5, 0)):
+        got = zeta.search(pos, 4, workers=64, tt_log2=24, split_depth=split_depth, waves_per_simd=waves)
+        assert (got["score"] == want).all(), (split_depth, waves)
+    with pytest.raises(zeta.ZetaError):
+        zeta.search(pos[:1], 4, split_depth=-1)
+
+
 def test_config3_depth6_golden(zeta, oracle):
     path = os.path.join(HERE, "golden", "c3_search6.npz")
     if not os.path.exists(path):

@@ -48,7 +48,8 @@ class SearchOpts(ctypes.Structure):
                 ("clear_tt", ctypes.c_int), ("max_nodes", ctypes.c_uint64), ("use_mlp", ctypes.c_int),
                 ("mlp_min_depth", ctypes.c_int), ("log_records", ctypes.c_int), ("log_shift", ctypes.c_int),
                 ("history", ctypes.c_void_p), ("history_len", ctypes.c_void_p),
-                ("pv", ctypes.c_void_p), ("pv_len", ctypes.c_int), ("order_hint", ctypes.c_int)]
+                ("pv", ctypes.c_void_p), ("pv_len", ctypes.c_int), ("order_hint", ctypes.c_int),
+                ("split_depth", ctypes.c_int), ("waves_per_simd", ctypes.c_int)]
 
 
 ZETA_HISTORY = 128
@@ -208,7 +209,8 @@ def perft_device(d_pos, n, depth, d_counts, nobulk=False):
 
 
 def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_mlp=False,
-           mlp_min_depth=3, log_records=False, log_shift=0, history=None, pv_len=0, order_hint=False):
+           mlp_min_depth=3, log_records=False, log_shift=0, history=None, pv_len=0, order_hint=False,
+           split_depth=0, waves_per_simd=0):
     """Fixed-depth alpha-beta + quiescence of every position (iterative
     deepening 1..depth).  Returns a structured array (score, bestmove, nodes,
     depth, status) -- the mirror of rootsearch's PV[0]/PV[1]/node count.
@@ -216,7 +218,9 @@ def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_ml
     positions played before it, newest first, for repetitions across the root.
     pv_len > 0: also return the principal variations (n x pv_len move codes).
     order_hint: search with the moves of the previous batch's table first
-    (zeta_search_opts.order_hint)."""
+    (zeta_search_opts.order_hint).  split_depth, waves_per_simd: the
+    autotuner's ABDADA split depth and persistent wavefronts per SIMD (0 =
+    default)."""
     ensure_init()
     pos = as_positions(pos)
     out = np.zeros(len(pos), dtype=RESULT_DTYPE)
@@ -234,7 +238,8 @@ def search(pos, depth, workers=1, tt_log2=26, clear_tt=True, max_nodes=0, use_ml
     o = SearchOpts(int(depth), int(workers), int(tt_log2), int(bool(clear_tt)), int(max_nodes),
                    int(bool(use_mlp)), int(mlp_min_depth), int(bool(log_records)), int(log_shift),
                    hist.ctypes.data if hist is not None else None, hlen.ctypes.data if hlen is not None else None,
-                   pv.ctypes.data if pv_len > 0 else None, int(pv_len), int(bool(order_hint)))
+                   pv.ctypes.data if pv_len > 0 else None, int(pv_len), int(bool(order_hint)),
+                   int(split_depth), int(waves_per_simd))
     _check(lib().zeta_search_batch(_ptr(pos), len(pos), ctypes.byref(o), _ptr(out)), "zeta_search_batch")
     return (out, pv) if pv_len > 0 else out
 

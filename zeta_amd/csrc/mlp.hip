@@ -3,21 +3,22 @@
 // ranking loss, Adam with FP32 master weights.  No reference counterpart
 // (the reference orders moves with fixed scores, src/zeta.cl:2059-2104).
 //
-// Training examples are pairs logged by the plain search at cut nodes: the
-// move that caused the beta cutoff and a move searched before it that did
-// not (or one never searched).  A step takes P pairs = B = 2P examples (rows
-// [0,P) the cutoff moves, rows [P,2P) the others) and minimises
+// Training examples are pairs logged by the search at cut nodes: the move
+// that caused the beta cutoff and another move of the node.  A step takes P
+// pairs = B = 2P examples (row 2p the cutoff move of pair p, row 2p+1 the
+// other) and minimises
 //   L = mean_p softplus(s(neg_p) - s(pos_p) + o_p / PLAIN_PER_LOGIT),
 // o_p the plain ordering's score difference of the pair: the search ranks
 // by plain score + PLAIN_PER_LOGIT x s, so s learns a correction of it.
-// All five products are one NT GEMM kernel, C[M][N] = sum_k A[m][k] Bt[n][k],
-// both operands BF16 with k contiguous, one 32x32 C tile per wavefront built
-// from v_mfma_f32_32x32x16_bf16:
-//   z1   [B][256]   = X     [B][928]  . W1t  [256][928]
-//   z2   [B][32]    = H1    [B][256]  . W2t  [32][256]
-//   gW2t [32][256]  = dZ2t  [32][B]   . H1t  [256][B]
-//   dH1  [B][256]   = dZ2   [B][32]   . W2   [256][32]
-//   gW1t [256][928] = dZ1t  [256][B]  . Xt   [928][B]
+// One step is a feature pass and five products of one NT GEMM kernel,
+// C[M][N] = sum_k A[m][k] Bt[n][k], both operands BF16 with k contiguous,
+// one 32x32 C tile per wavefront built from v_mfma_f32_32x32x16_bf16, each
+// with its consumer fused in as the epilogue:
+//   z1   [B][256]   = X     [B][928]  . W1t  [256][928]  -> h1 = relu(z1 + b1)
+//   z2   [B][32]    = H1    [B][256]  . W2t  [32][256]   -> scores, loss, dz2
+//   gW2t [32][256]  = dZ2t  [32][B]   . H1t  [256][B]    split over K
+//   dH1  [B][256]   = dZ2   [B][32]   . W2   [256][32]   -> dz1, b1 gradient
+//   gW1t [256][928] = dZ1t  [256][B]  . Xt   [928][B]    split over K
 #include <math.h>
 #include <string.h>
 
@@ -51,180 +52,216 @@ struct PairRec {  // one logged training pair (48 B)
 
 constexpr u16 BF_ONE = 0x3F80;
 
-// ---- NT GEMM on MFMA: one wavefront per 32x32 tile of C and slice of K.
-// The weight-gradient products have few output tiles and a long K (the
-// batch): their K is split over ksplit wavefronts per tile, which add their
-// partial tiles into C (zeroed before) with FP32 atomics.
+// ---- examples of a batch: pair p gives rows 2p (its cutoff move) and 2p+1
+// (the other move), so every 32-row tile of a product holds 16 whole pairs
+// and the loss is an epilogue of the layer-2 product.
+
+// One-hot features of 32 rows (16 pairs) per block, written densely in both
+// layouts (X [B][928] and Xt [928][B]) so that no clearing pass is needed:
+// the block collects a 32-bit row mask per feature in LDS, then stores 16-B
+// chunks of X rows and 64-B row segments of Xt.
+__global__ __launch_bounds__(256) void k_features(const PairRec *__restrict__ recs, const u32 *__restrict__ idx,
+                                                  int B, u16 *X, u16 *Xt) {
+  __shared__ u32 mask[NINP];
+  const int e0 = blockIdx.x * 32;
+  for (int f = threadIdx.x; f < NINP; f += blockDim.x) mask[f] = 0;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const int r = threadIdx.x, e = e0 + r;
+    const PairRec rc = recs[idx[e >> 1]];
+    const u32 m = (e & 1) ? rc.mneg : rc.mpos;
+    zc::Pos p;
+    p.b0 = rc.b0; p.b1 = rc.b1; p.b2 = rc.b2; p.b3 = rc.b3; p.cr = 0; p.stm = rc.stm & 1u;
+    const u32 bit = 1u << r;
+    u64 occ = p.b1 | p.b2 | p.b3;
+    while (occ) {
+      const int sq = zc::lsb(occ);
+      occ &= occ - 1;
+      atomicOr(&mask[feat_ps(zc::piece_at(p, sq), sq, p.stm)], bit);
+    }
+    atomicOr(&mask[feat_from(m, p.stm)], bit);
+    atomicOr(&mask[feat_to(m, p.stm)], bit);
+    atomicOr(&mask[feat_piece(m)], bit);
+    const int fc = feat_capt(m);
+    if (fc >= 0) atomicOr(&mask[fc], bit);
+  }
+  __syncthreads();
+  constexpr int CH = NINP / 8;  // 16-B chunks per X row
+  for (int c = threadIdx.x; c < 32 * CH; c += blockDim.x) {
+    const int r = c / CH, f0 = (c % CH) * 8;
+    u32 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      w[j] = (((mask[f0 + 2 * j] >> r) & 1u) ? BF_ONE : 0u) | (((mask[f0 + 2 * j + 1] >> r) & 1u) ? (u32)BF_ONE << 16 : 0u);
+    *(uint4 *)(X + (size_t)(e0 + r) * NINP + f0) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  for (int c = threadIdx.x; c < NINP * 4; c += blockDim.x) {
+    const int f = c >> 2, r0 = (c & 3) * 8;
+    const u32 mk = mask[f] >> r0;
+    u32 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      w[j] = (((mk >> (2 * j)) & 1u) ? BF_ONE : 0u) | (((mk >> (2 * j + 1)) & 1u) ? (u32)BF_ONE << 16 : 0u);
+    *(uint4 *)(Xt + (size_t)f * B + e0 + r0) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// ---- NT GEMM on MFMA, C[M][N] = sum_k A[m][k] Bt[n][k], one wavefront per
+// 32x32 tile of C (and K slice), with the consumer of each product fused in
+// as its epilogue.  Accumulator layout of v_mfma_f32_32x32x16_bf16: register
+// i of lane l holds C[8*(i/4) + 4*(l/32) + i%4][l%32].
+enum { E_STORE, E_SPLITK, E_L1ACT, E_HEAD, E_L1BACK };
+struct Epi {
+  float *C;                   // E_STORE / E_SPLITK output, ldc = N
+  const float *prm;           // master weights (biases, head)
+  float *Zf;                  // E_L1ACT: h1 (FP32) out;  E_L1BACK: h1 in
+  u16 *H, *Ht;                // E_L1ACT: h1 BF16 [M][N] and [N][M];  E_L1BACK: Ht = dz1 [N][M];  E_HEAD: dz2, dz2t
+  float *grad, *loss;
+  const PairRec *recs;
+  const u32 *idx;
+  int P;
+};
+
+__device__ __forceinline__ uint2 pack4(const u16 *v) {
+  return make_uint2((u32)v[0] | ((u32)v[1] << 16), (u32)v[2] | ((u32)v[3] << 16));
+}
+
+// E_SPLITK: the four wavefronts of a block take four K slices of one tile and
+// add their partial tiles in LDS; the block then adds the sum into C (zeroed
+// before) with FP32 atomics, ksplit/4 blocks per tile.
+template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int lda, const u16 *__restrict__ Bt,
-                                                 int ldb, float *__restrict__ C, int ldc, int M, int N, int K,
-                                                 int ksplit) {
+                                                 int ldb, int M, int N, int K, int ksplit, Epi ep) {
   const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int tn_count = N / 32;
-  const int tile = wave / ksplit, ks = wave % ksplit;
+  int tile, ks;
+  if (EPI == E_SPLITK) {
+    const int bpt = ksplit / 4;
+    tile = blockIdx.x / bpt;
+    ks = (blockIdx.x % bpt) * 4 + (threadIdx.x >> 6);
+  } else {
+    tile = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    ks = 0;
+  }
   const int tm = tile / tn_count, tn = tile % tn_count;
-  if (tm * 32 >= M) return;
-  const int kc = K / ksplit, k0 = ks * kc;
+  if (EPI != E_SPLITK && tm * 32 >= M) return;
+  // K slices of whole 16-steps; the last ones may be short or empty
+  const int kc = ((K / 16 + ksplit - 1) / ksplit) * 16, k0 = ks * kc;
+  const int kn = K - k0 < kc ? K - k0 : kc;
   const int r = tm * 32 + (lane & 31), c = tn * 32 + (lane & 31);
   const int kofs = (lane >> 5) * 8;
   const u16 *pa = A + (size_t)r * lda + kofs + k0;
   const u16 *pb = Bt + (size_t)c * ldb + kofs + k0;
   f32x16 acc = {};
-  for (int k = 0; k < kc; k += 16) {
+  for (int k = 0; k < kn; k += 16) {
     const bf16x8 a = *(const bf16x8 *)(pa + k);
     const bf16x8 b = *(const bf16x8 *)(pb + k);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
-  // accumulator layout: register i of lane l holds C[8*(i/4) + 4*(l/32) + i%4][l%32]
+  const int half = lane >> 5, col = tn * 32 + (lane & 31);
+  if (EPI == E_STORE) {
 #pragma unroll
-  for (int i = 0; i < 16; i++) {
-    const int row = tm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-    float *dst = C + (size_t)row * ldc + tn * 32 + (lane & 31);
-    if (ksplit == 1) *dst = acc[i];
-    else atomicAdd(dst, acc[i]);
-  }
-}
-
-// ---- one-hot features of each example, in both layouts (X and its transpose)
-__device__ __forceinline__ void set_feat(u16 *X, u16 *Xt, int row, int f, int B) {
-  X[(size_t)row * NINP + f] = BF_ONE;
-  Xt[(size_t)f * B + row] = BF_ONE;
-}
-__global__ void k_features(const PairRec *__restrict__ recs, const u32 *__restrict__ idx, int P, u16 *X, u16 *Xt) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int B = 2 * P;
-  if (e >= B) return;
-  const PairRec rc = recs[idx[e % P]];
-  const u32 m = e < P ? rc.mpos : rc.mneg;
-  zc::Pos p;
-  p.b0 = rc.b0; p.b1 = rc.b1; p.b2 = rc.b2; p.b3 = rc.b3; p.cr = 0; p.stm = rc.stm & 1u;
-  u64 occ = p.b1 | p.b2 | p.b3;
-  while (occ) {
-    const int sq = zc::lsb(occ);
-    occ &= occ - 1;
-    set_feat(X, Xt, e, feat_ps(zc::piece_at(p, sq), sq, p.stm), B);
-  }
-  set_feat(X, Xt, e, feat_from(m, p.stm), B);
-  set_feat(X, Xt, e, feat_to(m, p.stm), B);
-  set_feat(X, Xt, e, feat_piece(m), B);
-  const int fc = feat_capt(m);
-  if (fc >= 0) set_feat(X, Xt, e, fc, B);
-}
-
-// h1 = relu(z1 + b1): FP32 in place, BF16 copies in both layouts
-__global__ void k_l1_act(float *Z1, const float *__restrict__ b1, u16 *H1, u16 *H1t, int B) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B * NH1) return;
-  const int e = i / NH1, u = i % NH1;
-  float h = Z1[i] + b1[u];
-  h = h > 0.f ? h : 0.f;
-  Z1[i] = h;
-  const u16 hb = f2bf(h);
-  H1[i] = hb;
-  H1t[(size_t)u * B + e] = hb;
-}
-
-// layer 2 activation, layer 3, pairwise ranking loss and the gradients that
-// need no GEMM; one thread per pair (rows p and P+p).  The sums over pairs
-// (b2, W3, b3 gradients, the loss) are wave reductions, then one LDS slot per
-// wave and one global atomic per block and value.
-__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int i = 0; i < 16; i++) ep.C[(size_t)(tm * 32 + 8 * (i >> 2) + 4 * half + (i & 3)) * N + col] = acc[i];
+  } else if (EPI == E_SPLITK) {
+    __shared__ float red[3][16][64];
+    const int w = threadIdx.x >> 6;
+    if (w) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__global__ __launch_bounds__(256) void k_head(const float *__restrict__ Z2, const float *__restrict__ prm, int P,
-                                              const PairRec *__restrict__ recs, const u32 *__restrict__ idx,
-                                              u16 *dZ2, u16 *dZ2t, float *grad, float *loss_out) {
-  constexpr int NV = NH2 * 2 + 2;  // b2, W3, b3, loss
-  __shared__ float red[4][NV];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int B = 2 * P;
-  const float *b2 = prm + I_B2, *w3 = prm + I_W3;
-  float gb2[NH2], gw3[NH2], gb3 = 0.f, gl = 0.f;
-#pragma unroll
-  for (int o = 0; o < NH2; o++) gb2[o] = gw3[o] = 0.f;
-  if (p < P) {
-    float h[2][NH2], s[2];
-    for (int k = 0; k < 2; k++) {
-      const int e = p + k * P;
-      float acc = prm[I_B3];
-#pragma unroll
-      for (int o = 0; o < NH2; o++) {
-        float z = Z2[(size_t)e * NH2 + o] + b2[o];
-        z = z > 0.f ? z : 0.f;
-        h[k][o] = z;
-        acc += w3[o] * z;
-      }
-      s[k] = acc;
+      for (int i = 0; i < 16; i++) red[w - 1][i][lane] = acc[i];
     }
-    const float d = s[1] - s[0] + (float)pair_offset(recs[idx[p]].depth) / (float)PLAIN_PER_LOGIT;
-    const float sig = 1.f / (1.f + expf(-d));
-    const float l = d > 0.f ? d + log1pf(expf(-d)) : log1pf(expf(d));
-    const float ds[2] = {-sig / (float)P, sig / (float)P};
-    for (int k = 0; k < 2; k++) {
-      const int e = p + k * P;
-#pragma unroll
-      for (int o = 0; o < NH2; o++) {
-        const float dz = h[k][o] > 0.f ? ds[k] * w3[o] : 0.f;
-        const u16 db = f2bf(dz);
-        dZ2[(size_t)e * NH2 + o] = db;
-        dZ2t[(size_t)o * B + e] = db;
-        gb2[o] += dz;
-        gw3[o] += ds[k] * h[k][o];
-      }
-      gb3 += ds[k];
-    }
-    gl = l / (float)P;
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int o = 0; o < NH2; o++) {
-    const float a = wave_sum_f(gb2[o]), b = wave_sum_f(gw3[o]);
-    if (lane == 0) {
-      red[w][o] = a;
-      red[w][NH2 + o] = b;
-    }
-  }
-  gb3 = wave_sum_f(gb3);
-  gl = wave_sum_f(gl);
-  if (lane == 0) {
-    red[w][2 * NH2] = gb3;
-    red[w][2 * NH2 + 1] = gl;
-  }
-  __syncthreads();
-  const int nw = (int)(blockDim.x >> 6);
-  for (int i = threadIdx.x; i < NV; i += blockDim.x) {
-    float t = 0.f;
-    for (int k = 0; k < nw; k++) t += red[k][i];
-    if (i < NH2) atomicAdd(&grad[I_B2 + i], t);
-    else if (i < 2 * NH2) atomicAdd(&grad[I_W3 + i - NH2], t);
-    else if (i == 2 * NH2) atomicAdd(&grad[I_B3], t);
-    else atomicAdd(loss_out, t);
-  }
-}
-
-// dz1 = dh1 * relu'(h1): BF16 transposed copy for the W1 gradient; b1 gradient
-__global__ __launch_bounds__(256) void k_l1_back(const float *__restrict__ dH1, const float *__restrict__ H1f, int B,
-                                                 u16 *dZ1t, float *grad) {
-  // one block per hidden unit column strip: thread t handles examples t, t+256, ...
-  const int u = blockIdx.x;
-  float acc = 0.f;
-  for (int e = threadIdx.x; e < B; e += blockDim.x) {
-    const size_t i = (size_t)e * NH1 + u;
-    const float dz = H1f[i] > 0.f ? dH1[i] : 0.f;
-    dZ1t[(size_t)u * B + e] = f2bf(dz);
-    acc += dz;
-  }
-  __shared__ float red[256];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        atomicAdd(ep.C + (size_t)(tm * 32 + 8 * (i >> 2) + 4 * half + (i & 3)) * N + col,
+                  acc[i] + red[0][i][lane] + red[1][i][lane] + red[2][i][lane]);
+    }
+  } else if (EPI == E_L1ACT) {
+    // h1 = relu(z1 + b1): FP32 (kept for relu' in the backward pass), BF16 in both layouts
+    const float bb = ep.prm[I_B1 + col];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int rb = tm * 32 + 8 * g + 4 * half;
+      u16 hb[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        float h = acc[4 * g + j] + bb;
+        h = h > 0.f ? h : 0.f;
+        ep.Zf[(size_t)(rb + j) * N + col] = h;
+        hb[j] = f2bf(h);
+        ep.H[(size_t)(rb + j) * N + col] = hb[j];
+      }
+      *(uint2 *)(ep.Ht + (size_t)col * M + rb) = pack4(hb);
+    }
+  } else if (EPI == E_L1BACK) {
+    // dz1 = dh1 * relu'(h1): BF16 transposed for the W1 gradient; b1 gradient
+    float cs = 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const int rb = tm * 32 + 8 * g + 4 * half;
+      u16 db[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const float dz = ep.Zf[(size_t)(rb + j) * N + col] > 0.f ? acc[4 * g + j] : 0.f;
+        cs += dz;
+        db[j] = f2bf(dz);
+      }
+      *(uint2 *)(ep.Ht + (size_t)col * M + rb) = pack4(db);
+    }
+    cs += __shfl_xor(cs, 32, 64);
+    if (lane < 32) atomicAdd(ep.grad + I_B1 + col, cs);
+  } else if (EPI == E_HEAD) {
+    // N = 32: the tile holds all of layer 2.  h2 = relu(z2 + b2), s = w3.h2 + b3
+    // (sum over the 32 lanes of a half), then for the 16 pairs of the tile the
+    // loss softplus(s_neg - s_pos + o/PLAIN_PER_LOGIT), dz2 and the b2, W3,
+    // b3 gradients and the loss summed per wavefront
+    const int o = lane & 31;
+    const float b2o = ep.prm[I_B2 + o], w3o = ep.prm[I_W3 + o], b3 = ep.prm[I_B3];
+    float h[16], sc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const float z = acc[i] + b2o;
+      h[i] = z > 0.f ? z : 0.f;
+      float t = w3o * h[i];
+#pragma unroll
+      for (int x = 16; x > 0; x >>= 1) t += __shfl_xor(t, x, 64);
+      sc[i] = t + b3;
+    }
+    const float invP = 1.f / (float)ep.P;
+    float gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, gl = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int i = 2 * q;  // rows i, i+1 of this lane: an even row (cutoff move) and its pair
+      const int row = tm * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
+      const float d = sc[i + 1] - sc[i] + (float)pair_offset(ep.recs[ep.idx[row >> 1]].depth) / (float)PLAIN_PER_LOGIT;
+      const float sig = 1.f / (1.f + expf(-d));
+      const float l = d > 0.f ? d + log1pf(expf(-d)) : log1pf(expf(d));
+      const float ds0 = -sig * invP, ds1 = sig * invP;
+      const float dz0 = h[i] > 0.f ? ds0 * w3o : 0.f, dz1 = h[i + 1] > 0.f ? ds1 * w3o : 0.f;
+      const u16 d0 = f2bf(dz0), d1 = f2bf(dz1);
+      ep.H[(size_t)row * NH2 + o] = d0;
+      ep.H[(size_t)(row + 1) * NH2 + o] = d1;
+      *(u32 *)(ep.Ht + (size_t)o * M + row) = (u32)d0 | ((u32)d1 << 16);
+      gb2 += dz0 + dz1;
+      gw3 += ds0 * h[i] + ds1 * h[i + 1];
+      if (o == 0) {
+        gb3 += ds0 + ds1;
+        gl += l * invP;
+      }
+    }
+    gb2 += __shfl_xor(gb2, 32, 64);
+    gw3 += __shfl_xor(gw3, 32, 64);
+    gb3 += __shfl_xor(gb3, 32, 64);
+    gl += __shfl_xor(gl, 32, 64);
+    if (lane < 32) {
+      atomicAdd(ep.grad + I_B2 + o, gb2);
+      atomicAdd(ep.grad + I_W3 + o, gw3);
+    }
+    if (lane == 0) {
+      atomicAdd(ep.grad + I_B3, gb3);
+      atomicAdd(ep.loss, gl);
+    }
   }
-  if (threadIdx.x == 0) grad[I_B1 + u] = red[0];
 }
 
 // Adam on the internal layout; padded W1 columns stay zero
@@ -289,7 +326,7 @@ __global__ void k_sample(u32 *idx, int P, u64 nrec, u64 seed) {
 
 struct MlpState {
   zr::DevBuf prm, grad, m, v, w1t, w1f, w2t, w2;     // weights
-  zr::DevBuf X, Xt, Z1, H1, H1t, Z2, dZ2, dZ2t, dH1, dZ1t, idx, loss, recs_scratch;
+  zr::DevBuf X, Xt, Z1, H1, H1t, dZ2, dZ2t, dZ1t, idx, loss, recs_scratch;
   int cap_pairs = 0;
   bool ready = false;
   int step = 0;
@@ -313,9 +350,8 @@ int ensure_batch(int P) {
   int r;
   if ((r = zr::ensure(g_ms.X, B * NINP * 2)) || (r = zr::ensure(g_ms.Xt, B * NINP * 2)) ||
       (r = zr::ensure(g_ms.Z1, B * NH1 * 4)) || (r = zr::ensure(g_ms.H1, B * NH1 * 2)) ||
-      (r = zr::ensure(g_ms.H1t, B * NH1 * 2)) || (r = zr::ensure(g_ms.Z2, B * NH2 * 4)) ||
-      (r = zr::ensure(g_ms.dZ2, B * NH2 * 2)) || (r = zr::ensure(g_ms.dZ2t, B * NH2 * 2)) ||
-      (r = zr::ensure(g_ms.dH1, B * NH1 * 4)) || (r = zr::ensure(g_ms.dZ1t, B * NH1 * 2)) ||
+      (r = zr::ensure(g_ms.H1t, B * NH1 * 2)) || (r = zr::ensure(g_ms.dZ2, B * NH2 * 2)) ||
+      (r = zr::ensure(g_ms.dZ2t, B * NH2 * 2)) || (r = zr::ensure(g_ms.dZ1t, B * NH1 * 2)) ||
       (r = zr::ensure(g_ms.idx, (size_t)P * 4)) || (r = zr::ensure(g_ms.loss, 64)))
     return r;
   g_ms.cap_pairs = P;
@@ -324,15 +360,19 @@ int ensure_batch(int P) {
 
 inline unsigned nb(size_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
-// C = A . Bt^T.  accumulate = 1: C was zeroed and the K slices may add into
-// it, so a product with few output tiles splits its K to fill the device
-int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, float *C, int ldc, int M, int N, int K, int kid,
-         int accumulate = 0) {
+// C = A . Bt^T with epilogue EPI.  E_SPLITK: C was zeroed and K slices add
+// into it, so a product with few output tiles splits its K to fill the device
+template <int EPI>
+int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, int M, int N, int K, int kid, const Epi &ep) {
   const size_t tiles = (size_t)(M / 32) * (N / 32);
   int ksplit = 1;
-  while (accumulate && tiles * ksplit < 2048 && K % (32 * ksplit) == 0 && K / (2 * ksplit) >= 128) ksplit *= 2;
-  ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt, nb(tiles * ksplit * 64, 256), 256, 0, A, lda, Bt, ldb, C, ldc, M, N,
-          K, ksplit);
+  if (EPI == E_SPLITK) {
+    ksplit = 4;
+    while (tiles * ksplit < 2048 && K / (2 * ksplit) >= 128) ksplit *= 2;
+  }
+  if (M % 32 || N % 32 || K % 16) return zr::fail(ZETA_EINVAL, "mlp gemm: shape %d x %d x %d", M, N, K);
+  ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt<EPI>, nb(tiles * ksplit * 64, 256), 256, 0, A, lda, Bt, ldb, M, N, K,
+          ksplit, ep);
   return ZETA_OK;
 }
 
@@ -342,34 +382,41 @@ int fwd_bwd(const PairRec *recs, int P) {
   hipStream_t st = zr::ctx().stream;
   const int B = 2 * P;
   int r;
-  ZCHECK(hipMemsetAsync(g_ms.X.p, 0, (size_t)B * NINP * 2, st));
-  ZCHECK(hipMemsetAsync(g_ms.Xt.p, 0, (size_t)B * NINP * 2, st));
   ZCHECK(hipMemsetAsync(g_ms.grad.p, 0, NINT * 4, st));
   ZCHECK(hipMemsetAsync(g_ms.loss.p, 0, 4, st));
-  ZLAUNCH(zr::K_MLP_FWD, B, k_features, nb(B, 256), 256, 0, recs, (const u32 *)g_ms.idx.p, P, (u16 *)g_ms.X.p,
-          (u16 *)g_ms.Xt.p);
-  const float *prm = (const float *)g_ms.prm.p;
-  float *grad = (float *)g_ms.grad.p;
-  if ((r = gemm((const u16 *)g_ms.X.p, NINP, (const u16 *)g_ms.w1t.p, NINP, (float *)g_ms.Z1.p, NH1, B, NH1, NINP,
-                zr::K_MLP_FWD)))
+  const u32 *idx = (const u32 *)g_ms.idx.p;
+  ZLAUNCH(zr::K_MLP_FWD, B, k_features, B / 32, 256, 0, recs, idx, B, (u16 *)g_ms.X.p, (u16 *)g_ms.Xt.p);
+  Epi ep = {};
+  ep.prm = (const float *)g_ms.prm.p;
+  ep.grad = (float *)g_ms.grad.p;
+  ep.loss = (float *)g_ms.loss.p;
+  ep.recs = recs;
+  ep.idx = idx;
+  ep.P = P;
+  // z1 = X . W1t^T -> h1
+  ep.Zf = (float *)g_ms.Z1.p;
+  ep.H = (u16 *)g_ms.H1.p;
+  ep.Ht = (u16 *)g_ms.H1t.p;
+  if ((r = gemm<E_L1ACT>((const u16 *)g_ms.X.p, NINP, (const u16 *)g_ms.w1t.p, NINP, B, NH1, NINP, zr::K_MLP_FWD,
+                         ep)))
     return r;
-  ZLAUNCH(zr::K_MLP_FWD, B, k_l1_act, nb((size_t)B * NH1, 256), 256, 0, (float *)g_ms.Z1.p, prm + I_B1,
-          (u16 *)g_ms.H1.p, (u16 *)g_ms.H1t.p, B);
-  if ((r = gemm((const u16 *)g_ms.H1.p, NH1, (const u16 *)g_ms.w2t.p, NH1, (float *)g_ms.Z2.p, NH2, B, NH2, NH1,
-                zr::K_MLP_FWD)))
+  // z2 = H1 . W2t^T -> scores, loss, dz2 and the head's gradients
+  ep.H = (u16 *)g_ms.dZ2.p;
+  ep.Ht = (u16 *)g_ms.dZ2t.p;
+  if ((r = gemm<E_HEAD>((const u16 *)g_ms.H1.p, NH1, (const u16 *)g_ms.w2t.p, NH1, B, NH2, NH1, zr::K_MLP_BWD, ep)))
     return r;
-  ZLAUNCH(zr::K_MLP_BWD, P, k_head, nb(P, 256), 256, 0, (const float *)g_ms.Z2.p, prm, P, recs,
-          (const u32 *)g_ms.idx.p, (u16 *)g_ms.dZ2.p, (u16 *)g_ms.dZ2t.p, grad, (float *)g_ms.loss.p);
-  if ((r = gemm((const u16 *)g_ms.dZ2t.p, B, (const u16 *)g_ms.H1t.p, B, grad + I_W2T, NH1, NH2, NH1, B,
-                zr::K_MLP_BWD, 1)))
+  // gW2t = dZ2t . H1t^T
+  ep.C = (float *)g_ms.grad.p + I_W2T;
+  if ((r = gemm<E_SPLITK>((const u16 *)g_ms.dZ2t.p, B, (const u16 *)g_ms.H1t.p, B, NH2, NH1, B, zr::K_MLP_BWD, ep)))
     return r;
-  if ((r = gemm((const u16 *)g_ms.dZ2.p, NH2, (const u16 *)g_ms.w2.p, NH2, (float *)g_ms.dH1.p, NH1, B, NH1, NH2,
-                zr::K_MLP_BWD)))
+  // dh1 = dZ2 . W2^T -> dz1 (transposed), b1 gradient
+  ep.Ht = (u16 *)g_ms.dZ1t.p;
+  if ((r = gemm<E_L1BACK>((const u16 *)g_ms.dZ2.p, NH2, (const u16 *)g_ms.w2.p, NH2, B, NH1, NH2, zr::K_MLP_BWD,
+                          ep)))
     return r;
-  ZLAUNCH(zr::K_MLP_BWD, B, k_l1_back, NH1, 256, 0, (const float *)g_ms.dH1.p, (const float *)g_ms.Z1.p, B,
-          (u16 *)g_ms.dZ1t.p, grad);
-  if ((r = gemm((const u16 *)g_ms.dZ1t.p, B, (const u16 *)g_ms.Xt.p, B, grad + I_W1T, NINP, NH1, NINP, B,
-                zr::K_MLP_BWD, 1)))
+  // gW1t = dZ1t . Xt^T
+  ep.C = (float *)g_ms.grad.p + I_W1T;
+  if ((r = gemm<E_SPLITK>((const u16 *)g_ms.dZ1t.p, B, (const u16 *)g_ms.Xt.p, B, NH1, NINP, B, zr::K_MLP_BWD, ep)))
     return r;
   return ZETA_OK;
 }
@@ -461,8 +508,8 @@ const zm::Weights *mlp_weights() {
 
 void mlp_release() {
   zr::DevBuf *bufs[] = {&g_ms.prm, &g_ms.grad, &g_ms.m,   &g_ms.v,    &g_ms.w1t,  &g_ms.w1f, &g_ms.w2t, &g_ms.w2,
-                        &g_ms.X,   &g_ms.Xt,   &g_ms.Z1,  &g_ms.H1,   &g_ms.H1t,  &g_ms.Z2,  &g_ms.dZ2,
-                        &g_ms.dZ2t, &g_ms.dH1, &g_ms.dZ1t, &g_ms.idx, &g_ms.loss, &g_ms.recs_scratch};
+                        &g_ms.X,   &g_ms.Xt,   &g_ms.Z1,  &g_ms.H1,   &g_ms.H1t,  &g_ms.dZ2,
+                        &g_ms.dZ2t, &g_ms.dZ1t, &g_ms.idx, &g_ms.loss, &g_ms.recs_scratch};
   for (auto *b : bufs) zr::release(*b);
   g_ms.cap_pairs = 0;
   g_ms.ready = false;

@@ -55,7 +55,7 @@ constexpr s32 KILLER_SCORE = 1400;
 constexpr s32 COUNTER_SCORE = 1300;
 constexpr int NCOUNTER = 64 * 64;  // counter-move table entries per worker (LDS)
 constexpr u32 DEFERRED = 0x80000000u;  // move-stack flag: deferred by ABDADA (move codes use bits 0-29)
-constexpr int ABDADA_MIN_DEPTH = 2;
+constexpr int ABDADA_MIN_DEPTH = 2;  // default split depth (zeta_search_opts.split_depth)
 constexpr int POLL_EVERY = 64;
 
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte TT entry
@@ -105,6 +105,7 @@ struct SearchArgs {
   int nroots;
   int depth;
   int workers;       // at most this many workers on one root
+  int split_depth;   // ABDADA deferral / busy marking at depth - 1 >= split_depth
   u64 *tt;           // 2 words per entry
   u64 ttmask;        // entries - 1 (0 = no TT)
   const u64 *hint;   // read-only table in the TT's format whose moves go first (null = none)
@@ -1138,7 +1139,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
           child_hash(n, m, c, ch, cph);
           // ABDADA: defer a sibling that another worker is already searching
           bool cbusy = false;
-          if (a.workers > 1 && n.depth - 1 >= ABDADA_MIN_DEPTH) {
+          if (a.workers > 1 && n.depth - 1 >= a.split_depth) {
             const u32 bidx = (u32)(tt_key(ch, c.cr) >> 40) & a.busymask;
             u32 b = 0;
             if (lane == 0) b = __atomic_load_n(&a.busy[bidx], __ATOMIC_RELAXED);
@@ -1387,6 +1388,9 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   if (r) return r;
   if (n < 0 || (n > 0 && (!pos || !res)) || !opts) return zr::fail(ZETA_EINVAL, "search: bad arguments");
   if (opts->depth < 1 || opts->depth > 40) return zr::fail(ZETA_EINVAL, "search: depth %d out of range", opts->depth);
+  if (opts->split_depth < 0 || opts->split_depth > 40 || opts->waves_per_simd < 0 || opts->waves_per_simd > 16)
+    return zr::fail(ZETA_EINVAL, "search: split_depth %d / waves_per_simd %d out of range", opts->split_depth,
+                    opts->waves_per_simd);
   if (n == 0) return ZETA_OK;
   const int workers = opts->workers < 1 ? 1 : (opts->workers > 1024 ? 1024 : opts->workers);
   hipStream_t st = zr::ctx().stream;
@@ -1431,6 +1435,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
 
   int per_cu = 0;
   ZCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_search, 64, 0));
+  if (opts->waves_per_simd > 0 && 4 * opts->waves_per_simd < per_cu) per_cu = 4 * opts->waves_per_simd;
   if (per_cu < 1) per_cu = 1;
   const u64 cap = (u64)per_cu * (u64)zr::ctx().cu_count;
   u64 want = (u64)n * (u64)workers;
@@ -1441,6 +1446,7 @@ extern "C" int zeta_search_batch(const zeta_pos *pos, int n, const zeta_search_o
   sa.nroots = n;
   sa.depth = opts->depth;
   sa.workers = workers;
+  sa.split_depth = opts->split_depth > 0 ? opts->split_depth : ABDADA_MIN_DEPTH;
   sa.tt = (u64 *)g_ss.tt.p;
   sa.ttmask = tt_entries ? tt_entries - 1 : 0;
   sa.hint = opts->order_hint ? (const u64 *)g_ss.hint.p : nullptr;
@@ -1783,6 +1789,7 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   SearchArgs sa;
   memset(&sa, 0, sizeof(sa));
   sa.workers = S.workers;
+  sa.split_depth = ABDADA_MIN_DEPTH;
   sa.tt = (u64 *)g_ss.tt.p;
   sa.ttmask = tt_entries ? tt_entries - 1 : 0;
   sa.busy = (u32 *)S.busy.p;

@@ -54,6 +54,9 @@ int workers_of() {
 
 }  // namespace
 
+int32_t zeta_split_depth = 0;
+int32_t zeta_waves_per_simd = 0;
+
 extern "C" bool cl_init_device(char *kernelname) {
   (void)kernelname;  // one library serves "perft_gpu" and "alphabeta_gpu"
   if (g.up) return true;
@@ -131,6 +134,8 @@ extern "C" bool cl_run_alphabeta(bool stm, int32_t depth, uint64_t nodes) {
     o.history_len = &g.hist_len;
     o.pv = g.pv;
     o.pv_len = MAXPLY - 1;
+    o.split_depth = zeta_split_depth;
+    o.waves_per_simd = zeta_waves_per_simd;
     zeta_result r;
     memset(&r, 0, sizeof r);
     const int rc = zeta_search_batch(&g.root, 1, &o, &r);

@@ -561,6 +561,9 @@ bool read_and_init_config(const char *path) {
     sscanf(line, "tt2_memory: %" SCNu64 ";", &tt2_memory);
     sscanf(line, "opencl_platform_id: %d;", &opencl_platform_id);
     sscanf(line, "opencl_device_id: %d;", &opencl_device_id);
+    // this build's launch knobs (written by --guessconfig; absent = defaults)
+    sscanf(line, "split_depth: %d;", &zeta_split_depth);
+    sscanf(line, "waves_per_simd: %d;", &zeta_waves_per_simd);
   }
   fclose(f);
   MaxNodes = (uint64_t)nodes_per_second;
@@ -577,68 +580,125 @@ bool engineinits() {
   return GLOBAL_BOARD && GLOBAL_HASHHISTORY && COUNTERS && PV;
 }
 
-// Autotuner (replaces --guessconfig[x], src/clconfig.c): measures search
-// throughput for a few worker counts per root on seeded positions and writes
-// config.txt with the best one, the TT size and the measured nodes per second.
+// Autotuner (replaces --guessconfig[x], src/clconfig.c:878-960, which
+// doubles threadsY while the measured nodes per second still rise by a
+// margin).  Coordinate descent over the search's launch knobs on seeded
+// playout positions, each knob keeping the value with the shortest total
+// time to depth (more workers always raise nodes/s, not always the time to
+// depth):
+//   1. workers per root (threadsX): each position searched alone, as the
+//      engine searches (ID 1..depth, TT cleared per root);
+//   2. ABDADA split depth (split_depth) at the chosen worker count;
+//   3. persistent wavefronts per SIMD (waves_per_simd) on a batch of roots
+//      large enough to fill the device (a single root never does).
+// Writes config.txt with the winners, the TT size and the measured nodes per
+// second, then reads it back.
 bool guess_config(bool extended) {
   zeta_config cfg;
   memset(&cfg, 0, sizeof cfg);
   cfg.device = opencl_device_id;
-  cfg.tt_log2 = 24;
+  cfg.tt_log2 = 26;  // tt1_memory 1024 MB
   int rc = zeta_init(&cfg);
-  if (rc) {
+  auto fail = [&](const char *what) {
     char msg[512];
     zeta_last_error(msg, sizeof msg);
-    out("Error (%s): --guessconfig\n", msg);
+    out("Error (%s: %s): --guessconfig\n", what, msg);
+    zeta_shutdown();
     return false;
-  }
-  // the engine searches one root at a time: each position is searched alone
-  // (ID 1..depth, TT cleared), and the worker count with the shortest total
-  // time to depth wins (more workers always raise nodes/s, not always the
-  // time to depth)
-  const int npos = extended ? 16 : 4;
+  };
+  if (rc) return fail("zeta_init");
+  const int npos = extended ? 16 : 4, depth = extended ? 6 : 5;
   std::vector<zeta_pos> pos(npos);
-  rc = zeta_playouts(3, npos, 20, 60, pos.data());
+  if ((rc = zeta_playouts(3, npos, 20, 60, pos.data()))) return fail("zeta_playouts");
   std::vector<zeta_result> res(npos);
-  int best_w = 1;
-  double best_t = 0, best_nps = 0;
-  const std::vector<int> cands = extended ? std::vector<int>{1, 2, 4, 8, 16, 32, 64} : std::vector<int>{1, 4, 16, 64};
-  for (int w : cands) {
+  auto opts_of = [&](int w, int sd, int wps, int d) {
     zeta_search_opts o;
     memset(&o, 0, sizeof o);
-    o.depth = extended ? 6 : 5;
+    o.depth = d;
     o.workers = w;
-    o.tt_log2 = 24;
+    o.tt_log2 = 26;
     o.clear_tt = 1;
-    double total = 0;
+    o.split_depth = sd;
+    o.waves_per_simd = wps;
+    return o;
+  };
+  // one root at a time; total seconds and nodes/s
+  auto single = [&](int w, int sd, double &secs, double &nps) {
+    const zeta_search_opts o = opts_of(w, sd, 0, depth);
+    secs = 0;
     uint64_t nodes = 0;
-    for (int i = 0; i < npos && !rc; i++) {
+    for (int i = 0; i < npos; i++) {
       const double t0 = now_ms();
-      rc = zeta_search_batch(&pos[i], 1, &o, &res[i]);
-      total += (now_ms() - t0) / 1000;
+      if ((rc = zeta_search_batch(&pos[i], 1, &o, &res[i]))) return false;
+      secs += (now_ms() - t0) / 1000;
       nodes += res[i].nodes;
     }
-    if (rc) break;
-    const double nps = (double)nodes / (total > 0 ? total : 1e-9);
-    out("#> workers per root %d: time to depth %d over %d positions %.3f s, %.0f nodes/s\n", w, o.depth, npos,
-        total, nps);
-    if (best_t == 0 || total < best_t) {
-      best_t = total;
-      best_nps = nps;
-      best_w = w;
+    nps = (double)nodes / (secs > 0 ? secs : 1e-9);
+    return true;
+  };
+  int best_w = 1, best_sd = 2, best_wps = 0;
+  double best_t = 0, best_nps = 0;
+  const std::vector<int> wc = extended ? std::vector<int>{1, 2, 4, 8, 16, 32, 64} : std::vector<int>{1, 4, 16, 64};
+  for (int w : wc) {
+    double t, nps;
+    if (!single(w, best_sd, t, nps)) return fail("workers");
+    out("#> workers per root %d: time to depth %d over %d positions %.3f s, %.0f nodes/s\n", w, depth, npos, t, nps);
+    if (best_t == 0 || t < best_t) best_t = t, best_nps = nps, best_w = w;
+  }
+  if (best_w > 1) {
+    const std::vector<int> sc = extended ? std::vector<int>{1, 2, 3, 4, 5} : std::vector<int>{1, 2, 3};
+    for (int sd : sc) {
+      if (sd == 2) continue;  // measured above
+      double t, nps;
+      if (!single(best_w, sd, t, nps)) return fail("split_depth");
+      out("#> split depth %d (workers %d): time to depth %d over %d positions %.3f s, %.0f nodes/s\n", sd, best_w,
+          depth, npos, t, nps);
+      if (t < best_t) best_t = t, best_nps = nps, best_sd = sd;
+    }
+  }
+  {
+    // a batch of roots that asks for more wavefronts than the device holds
+    const int nb = extended ? 1024 : 256;
+    std::vector<zeta_pos> bp(nb);
+    std::vector<zeta_result> br(nb);
+    if ((rc = zeta_playouts(5, nb, 20, 60, bp.data()))) return fail("zeta_playouts");
+    double bt = 0;
+    for (int wps : {2, 1}) {
+      const zeta_search_opts o = opts_of(best_w < 8 ? 8 : best_w, best_sd, wps, depth - 1);
+      const double t0 = now_ms();
+      if ((rc = zeta_search_batch(bp.data(), nb, &o, br.data()))) return fail("waves_per_simd");
+      const double t = (now_ms() - t0) / 1000;
+      out("#> waves per SIMD %d: batch of %d roots to depth %d in %.3f s\n", wps, nb, depth - 1, t);
+      if (bt == 0 || t < bt) bt = t, best_wps = wps;
     }
   }
   zeta_shutdown();
-  if (rc) return false;
   FILE *f = fopen(configfile, "w");
-  if (!f) return false;
+  if (!f) {
+    out("Error (cannot write %s): --guessconfig\n", configfile);
+    return false;
+  }
   fprintf(f, "threadsX: %d;\nthreadsY: 1;\nnodes_per_second: %" PRIu64 ";\ntt1_memory: 1024;\ntt2_memory: 0;\n"
-             "opencl_platform_id: 0;\nopencl_device_id: %d;\n",
-          best_w, (uint64_t)best_nps, opencl_device_id);
+             "opencl_platform_id: 0;\nopencl_device_id: %d;\nsplit_depth: %d;\nwaves_per_simd: %d;\n",
+          best_w, (uint64_t)best_nps, opencl_device_id, best_sd, best_wps);
+  fprintf(f, "# threadsX          workers (ABDADA wavefronts) per root\n"
+             "# threadsY          multiplier of threadsX (1)\n"
+             "# nodes_per_second  measured search speed, seeds the time control\n"
+             "# tt1_memory        transposition table in MB (2^n x 16 B entries)\n"
+             "# tt2_memory        unused (ABDADA keeps its busy table beside the TT)\n"
+             "# opencl_device_id  HIP device ordinal\n"
+             "# split_depth       ABDADA defers busy siblings at nodes with this much depth left\n"
+             "# waves_per_simd    persistent search wavefronts per SIMD\n");
   fclose(f);
-  out("#> config written to %s (threadsX %d, nodes_per_second %" PRIu64 ")\n", configfile, best_w,
-      (uint64_t)best_nps);
-  return true;
+  out("#> config written to %s (threadsX %d, split_depth %d, waves_per_simd %d, nodes_per_second %" PRIu64 ")\n",
+      configfile, best_w, best_sd, best_wps, (uint64_t)best_nps);
+  // read back what a later start will read
+  threadsX = 0;
+  zeta_split_depth = zeta_waves_per_simd = -1;
+  if (!read_and_init_config(configfile)) return false;
+  out("#> config read back: threadsX %" PRIu64 ", split_depth %d, waves_per_simd %d, nodes_per_second %" PRIi64
+      "\n", threadsX, zeta_split_depth, zeta_waves_per_simd, nodes_per_second);
+  return threadsX == (uint64_t)best_w && zeta_split_depth == best_sd && zeta_waves_per_simd == best_wps;
 }
 
 // The two position sets of the reference's SMP benchmarks (the same
