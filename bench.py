@@ -37,9 +37,13 @@ HBM_PEAK = 8.0e12
 #   traffic per position.
 VALU_LANE_INSTS_PER_POSITION = 1008.5
 HBM_BYTES_PER_POSITION = 12.85
-# k_search: one node per wavefront; SQ_INSTS_VALU / nodes over the depth-5
-# search leg (profiles/search_c3_pmc.md)
-VALU_WAVE_INSTS_PER_NODE = 915.8
+# k_search: one node per wavefront; SQ_INSTS_VALU and SQ_INSTS_SALU / nodes
+# over the depth-6 search leg (profiles/search_c3_pmc.md).  A CU's arbiter
+# issues at most one scalar instruction per cycle (each SIMD's waves get one
+# SALU slot every 4 cycles), so the SALU issue peak is 256 CU x 2.4 GHz.
+VALU_WAVE_INSTS_PER_NODE = 449.2
+SALU_INSTS_PER_NODE = 867.6
+SALU_PEAK = 256 * 2.4e9
 # the ordering network: 908 -> 256 -> 32 -> 1, dense FLOP of one scored move
 # (2 x (908 x 256 + 256 x 32 + 32)); a training pair scores two moves and its
 # step costs about three forward passes (forward, two backward products)
@@ -53,7 +57,7 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def c4_leg(zeta_amd, depth=7, workers=32, mlp_min_depth=3):
+def c4_leg(zeta_amd, depth=7, workers=128, waves_per_simd=1, mlp_min_depth=3):
     """Learned vs plain move ordering: 1024 held-out playout positions (seed
     40, 20-60 plies), depth 7, one batch each, TT 2^26 cleared before each;
     then the same search ordered by the best moves the completed search left
@@ -61,7 +65,8 @@ def c4_leg(zeta_amd, depth=7, workers=32, mlp_min_depth=3):
     from zeta_amd import learned as L
     pos = zeta_amd.playouts(40, 1024, 20, 60)
     out = {"workload": "1024 playout positions seed 40, 20-60 plies, depth %d + quiescence, TT 2^26 x 16 B, "
-                       "%d workers max per root" % (depth, workers), "mlp_min_depth": mlp_min_depth,
+                       "%d workers max per root, %d wavefront(s) per SIMD" % (depth, workers, waves_per_simd),
+           "mlp_min_depth": mlp_min_depth,
            "network": os.path.relpath(NET_PATH, ROOT)}
     ref = None
     for mode in ("plain", "learned", "hinted"):
@@ -69,7 +74,8 @@ def c4_leg(zeta_amd, depth=7, workers=32, mlp_min_depth=3):
             L.load(NET_PATH)
         t = time.perf_counter()
         r = zeta_amd.search(pos, depth, workers=workers, tt_log2=26, clear_tt=True, use_mlp=(mode == "learned"),
-                            mlp_min_depth=mlp_min_depth, order_hint=(mode == "hinted"))
+                            mlp_min_depth=mlp_min_depth, order_hint=(mode == "hinted"),
+                            waves_per_simd=waves_per_simd)
         dt = time.perf_counter() - t
         ctr = zeta_amd.search_counters()
         nodes = int(r["nodes"].sum())
@@ -187,7 +193,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-search", action="store_true", help="skip the fixed-depth search leg")
     ap.add_argument("--search-depth", type=int, default=6)
-    ap.add_argument("--search-workers", type=int, default=32)
+    ap.add_argument("--search-workers", type=int, default=128)
+    ap.add_argument("--search-waves", type=int, default=1, help="persistent search wavefronts per SIMD (0 = all "
+                    "that fit)")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-vs-plain ordering leg (depth 7)")
     ap.add_argument("--no-c5", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-mlp", action="store_true", help="skip the ordering network's training leg")
@@ -303,21 +311,32 @@ def main():
     if not args.no_search:
         spos = zeta_amd.playouts(3, 1024, 20, 60)
         t2 = time.perf_counter()
-        res = zeta_amd.search(spos, args.search_depth, workers=args.search_workers, tt_log2=26, clear_tt=True)
+        res = zeta_amd.search(spos, args.search_depth, workers=args.search_workers, tt_log2=26, clear_tt=True,
+                              waves_per_simd=args.search_waves)
         ttd = time.perf_counter() - t2
         ctr = zeta_amd.search_counters()
         nodes = int(res["nodes"].sum())
         tt_bytes = 16 * (ctr["tt_probes"] + ctr["tt_stores"])
         search = {"workload": "1024 playout positions seed 3, 20-60 plies, depth %d + quiescence, TT 2^26 x 16 B, "
-                              "%d workers max per root" % (args.search_depth, args.search_workers),
+                              "%d workers max per root, %d wavefront(s) per SIMD"
+                              % (args.search_depth, args.search_workers, args.search_waves),
                   "time_to_depth_s": round(ttd, 3), "nodes": nodes, "nodes_per_s": nodes / ttd,
                   "score_checksum": int(res["score"].astype(np.int64).sum()),
                   "tt_GBps": round(tt_bytes / ttd / 1e9, 3), "tt_frac_of_hbm": round(tt_bytes / ttd / HBM_PEAK, 6),
-                  "roofline": {"bound": "valu", "unit": "T VALU lane-instructions/s",
+                  "roofline": {
+                      "bound": "salu", "unit": "G SALU instructions/s",
+                      "achieved": round(ctr["worker_nodes"] * SALU_INSTS_PER_NODE / ttd / 1e9, 2),
+                      "peak": round(SALU_PEAK / 1e9, 1),
+                      "frac": round(ctr["worker_nodes"] * SALU_INSTS_PER_NODE / ttd / SALU_PEAK, 4),
+                      "salu_insts_per_node": SALU_INSTS_PER_NODE,
+                      "valu": {"unit": "T VALU lane-instructions/s",
                                "achieved": round(ctr["worker_nodes"] * VALU_WAVE_INSTS_PER_NODE * 64 / ttd / 1e12, 3),
                                "peak": round(VALU_PEAK / 1e12, 2),
-                               "frac": round(ctr["worker_nodes"] * VALU_WAVE_INSTS_PER_NODE * 64 / ttd / VALU_PEAK, 4),
+                               "frac": round(ctr["worker_nodes"] * VALU_WAVE_INSTS_PER_NODE * 64 / ttd / VALU_PEAK,
+                                             4),
                                "valu_wave_insts_per_node": VALU_WAVE_INSTS_PER_NODE},
+                      "note": "issue fractions over the time to depth; one wavefront walks one tree, so the kernel "
+                              "is latency-bound and both fractions stay well below 1"},
                   **ctr}
 
     if search is not None:

@@ -189,6 +189,11 @@ def test_config3_depth6_golden(zeta, oracle):
     bad = np.nonzero(got["score"] != gold["score6"])[0]
     print("depth 6 mismatches", len(bad), "nodes", int(got["nodes"].sum()), flush=True)
     assert len(bad) == 0
+    # the bench's launch shape: up to 128 workers per root, one wavefront per SIMD
+    got = zeta.search(pos, 6, workers=128, tt_log2=26, clear_tt=True, waves_per_simd=1)
+    bad = np.nonzero(got["score"] != gold["score6"])[0]
+    print("depth 6 (128 workers) mismatches", len(bad), "nodes", int(got["nodes"].sum()), flush=True)
+    assert len(bad) == 0 and (got["status"] == 0).all()
 
 
 def _shuffle_games(oracle, n, seed):

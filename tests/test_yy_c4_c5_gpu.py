@@ -2,7 +2,8 @@
 workloads.
 
 Learned ordering: 1024 held-out playout positions (seed 40, 20-60 plies) at
-depth 7, one batch each, up to 32 workers per root, TT 2^26: with the
+depth 7, one batch each, up to 128 workers per root on one wavefront per
+SIMD (the autotuned launch shape), TT 2^26: with the
 committed network (zeta_amd/order_net.npz) every root score equals the plain
 ordering's (alpha-beta's value does not depend on move order).
 
@@ -27,7 +28,7 @@ def _c4_plain(zeta):
     if "c4_plain" not in _cache:
         pos = zeta.playouts(40, 1024, 20, 60)
         t = time.perf_counter()
-        r = zeta.search(pos, 7, workers=32, tt_log2=26, clear_tt=True)
+        r = zeta.search(pos, 7, workers=128, tt_log2=26, clear_tt=True, waves_per_simd=1)
         _cache["c4_plain"] = (pos, r, time.perf_counter() - t)
     return _cache["c4_plain"]
 
@@ -44,7 +45,8 @@ def test_c4_learned_ordering_identical_scores(zeta):
     pos, plain, dt_plain = _c4_plain(zeta)
     L.load(NET)
     t = time.perf_counter()
-    r = zeta.search(pos, 7, workers=32, tt_log2=26, clear_tt=True, use_mlp=True, mlp_min_depth=3)
+    r = zeta.search(pos, 7, workers=128, tt_log2=26, clear_tt=True, use_mlp=True, mlp_min_depth=3,
+                    waves_per_simd=1)
     dt = time.perf_counter() - t
     print(json.dumps({"plain_s": round(dt_plain, 2), "learned_s": round(dt, 2),
                       "plain_nodes": int(plain["nodes"].sum()), "learned_nodes": int(r["nodes"].sum()),

@@ -96,7 +96,7 @@ def train(steps=3000, pairs_per_step=2048, lr=2e-4, seed=1, init_seed=1, held=No
             "seconds": time.perf_counter() - t, "loss_first": first, "heldout_best": best, "curve": curve}
 
 
-def evaluate(seed=40, n=1024, depth=7, workers=32, mlp_min_depth=3, tt_log2=26):
+def evaluate(seed=40, n=1024, depth=7, workers=128, mlp_min_depth=3, tt_log2=26, waves_per_simd=1):
     """Plain vs learned ordering on held-out positions; identical root scores required."""
     pos = playouts(seed, n, 20, 60)
     out = {}
@@ -104,7 +104,7 @@ def evaluate(seed=40, n=1024, depth=7, workers=32, mlp_min_depth=3, tt_log2=26):
     for mode in ("plain", "learned"):
         t = time.perf_counter()
         r = search(pos, depth, workers=workers, tt_log2=tt_log2, clear_tt=True, use_mlp=(mode == "learned"),
-                   mlp_min_depth=mlp_min_depth)
+                   mlp_min_depth=mlp_min_depth, waves_per_simd=waves_per_simd)
         dt = time.perf_counter() - t
         res[mode] = r
         out[mode] = {"seconds": dt, "nodes": int(r["nodes"].sum()), **search_counters()}
@@ -113,7 +113,7 @@ def evaluate(seed=40, n=1024, depth=7, workers=32, mlp_min_depth=3, tt_log2=26):
     out["node_reduction"] = 1.0 - out["learned"]["nodes"] / max(out["plain"]["nodes"], 1)
     out["time_reduction"] = 1.0 - out["learned"]["seconds"] / max(out["plain"]["seconds"], 1e-9)
     out["config"] = {"positions": n, "seed": seed, "depth": depth, "workers": workers,
-                     "mlp_min_depth": mlp_min_depth, "tt_log2": tt_log2}
+                     "waves_per_simd": waves_per_simd, "mlp_min_depth": mlp_min_depth, "tt_log2": tt_log2}
     out["scores"] = res["plain"]["score"].tolist()
     return out
 
@@ -158,7 +158,7 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--eval-positions", type=int, default=1024)
     ap.add_argument("--eval-depth", type=int, default=7)
-    ap.add_argument("--eval-workers", type=int, default=32)
+    ap.add_argument("--eval-workers", type=int, default=128)
     ap.add_argument("--mlp-min-depth", type=int, default=3)
     ap.add_argument("--checkpoint", default=None)
     ap.add_argument("--load", default=None)
