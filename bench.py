@@ -130,15 +130,19 @@ def mlp_leg(zeta_amd, steps=400, pairs_per_step=2048):
                          "note": "dense FLOP over the summed time of the forward and backward kernels"}}
 
 
-def c5_leg(zeta_amd, rate=2000.0, n=48, max_seconds=60.0, workers=32):
+def c5_leg(zeta_amd, rate=2000.0, n=48, max_seconds=60.0, workers=64, grid=1024):
     """Serving: a Poisson stream (seed 5) of n requests at `rate`, depths
     uniform in 4-9, plain then learned ordering, each stream capped at
-    max_seconds (requests still running then are cancelled and reported)."""
+    max_seconds (requests still running then are cancelled and reported);
+    up to `workers` wavefronts per request on a persistent grid of `grid`
+    wavefronts (one per SIMD)."""
     from zeta_amd import learned as L
     from zeta_amd import serving as S
     L.load(NET_PATH)
     rep = S.compare(rate, n, seed=5, dmin=4, dmax=9, slots=256, workers=workers, max_seconds=max_seconds,
-                    mlp_min_depth=3, progress=lambda d: log("C5 " + json.dumps(d)))
+                    mlp_min_depth=3, grid=grid, progress=lambda d: log("C5 " + json.dumps(d)))
+    rep["workers_per_request"] = workers
+    rep["grid_wavefronts"] = grid
     for mode in ("plain", "learned"):
         rep[mode].pop("by_depth", None)
     return rep

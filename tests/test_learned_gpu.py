@@ -23,11 +23,19 @@ def test_trained_network_ranks_better_than_plain_at_identical_scores(zeta):
     from zeta_amd import learned as L
     # held-out pairs (seed 41) and training pairs (seed 4) from the plain
     # search with one worker per root (no ABDADA timing in which cut nodes
-    # are logged), in a fixed order (the training samples by index)
-    L.generate_records(seed=41, n=256, depth=4, workers=1, capacity=65536, chunk=256)
+    # are logged) at a fixed sampling rate, all of them (a full buffer would
+    # keep whichever pairs came first), in a fixed order (the training
+    # samples by index); the held-out set is every k-th pair of all 256
+    # positions' pairs
+    cap = 4_000_000  # 192 MB of records: no overflow at these rates
+    hrec = L.generate_records(seed=41, n=256, depth=4, workers=1, capacity=cap, chunk=256, log_shift=3)
+    print(json.dumps({"heldout_pairs_logged": hrec["pairs"]}), flush=True)
+    assert hrec["pairs"] < cap
     held = _sorted(zeta.records_get())
-    held = held[:(min(len(held), 8192) // 16) * 16]
-    rec = L.generate_records(seed=4, n=1024, depth=4, workers=1, capacity=400_000, chunk=1024)
+    held = held[::max(1, len(held) // 8192)][:8192]
+    held = held[:(len(held) // 16) * 16]
+    rec = L.generate_records(seed=4, n=1024, depth=4, workers=1, capacity=cap, chunk=1024, log_shift=5)
+    assert rec["pairs"] < cap
     zeta.records_put(_sorted(zeta.records_get()))
     print(json.dumps(rec), flush=True)
     assert rec["pairs"] > 20_000 and len(held) >= 1024
@@ -37,13 +45,13 @@ def test_trained_network_ranks_better_than_plain_at_identical_scores(zeta):
     loss_plain, _ = zeta.mlp_grad(held)
     off = held["depth"].view(np.int32) >> 8
     assert abs(loss_plain - float(np.mean(np.logaddexp(0, off / 100.0)))) < 1e-3
-    tr = L.train(steps=1000, pairs_per_step=2048, lr=2e-4, seed=3, init_seed=11, held=held)
+    tr = L.train(steps=2000, pairs_per_step=2048, lr=2e-4, seed=3, init_seed=11, held=held)
     loss_trained, _ = zeta.mlp_grad(held)
     print(json.dumps({**tr, "heldout_loss_plain": loss_plain, "heldout_loss_trained": loss_trained}), flush=True)
     # plain score + the network ranks pairs from positions it never saw
-    # better than the plain score alone (held-out loss ratio 0.969 measured
-    # with these seeds; the gain over the reference's ordering is small)
-    assert loss_trained < 0.985 * loss_plain
+    # better than the plain score alone (held-out loss ratio 0.92 after 1000
+    # steps, still falling, measured with these seeds)
+    assert loss_trained < 0.95 * loss_plain
 
     held_pos = zeta.playouts(40, 256, 20, 60)
     plain = zeta.search(held_pos, 5, workers=4, tt_log2=24, clear_tt=True)

@@ -58,32 +58,36 @@ constexpr u16 BF_ONE = 0x3F80;
 
 // One-hot features of 32 rows (16 pairs) per block, written densely in both
 // layouts (X [B][928] and Xt [928][B]) so that no clearing pass is needed:
-// the block collects a 32-bit row mask per feature in LDS, then stores 16-B
-// chunks of X rows and 64-B row segments of Xt.
+// the block (256 threads) collects a 32-bit row mask per feature in LDS,
+// then stores 16-B chunks of X rows and 64-B row segments of Xt.
 __global__ __launch_bounds__(256) void k_features(const PairRec *__restrict__ recs, const u32 *__restrict__ idx,
                                                   int B, u16 *X, u16 *Xt) {
   __shared__ u32 mask[NINP];
   const int e0 = blockIdx.x * 32;
   for (int f = threadIdx.x; f < NINP; f += blockDim.x) mask[f] = 0;
   __syncthreads();
-  if (threadIdx.x < 32) {
-    const int r = threadIdx.x, e = e0 + r;
+  {
+    // eight threads per row, one rank of the board each (the first one also
+    // the move's features)
+    const int r = threadIdx.x >> 3, rank = threadIdx.x & 7, e = e0 + r;
     const PairRec rc = recs[idx[e >> 1]];
-    const u32 m = (e & 1) ? rc.mneg : rc.mpos;
     zc::Pos p;
     p.b0 = rc.b0; p.b1 = rc.b1; p.b2 = rc.b2; p.b3 = rc.b3; p.cr = 0; p.stm = rc.stm & 1u;
     const u32 bit = 1u << r;
-    u64 occ = p.b1 | p.b2 | p.b3;
+    u64 occ = (p.b1 | p.b2 | p.b3) & (0xFFull << (8 * rank));
     while (occ) {
       const int sq = zc::lsb(occ);
       occ &= occ - 1;
       atomicOr(&mask[feat_ps(zc::piece_at(p, sq), sq, p.stm)], bit);
     }
-    atomicOr(&mask[feat_from(m, p.stm)], bit);
-    atomicOr(&mask[feat_to(m, p.stm)], bit);
-    atomicOr(&mask[feat_piece(m)], bit);
-    const int fc = feat_capt(m);
-    if (fc >= 0) atomicOr(&mask[fc], bit);
+    if (rank == 0) {
+      const u32 m = (e & 1) ? rc.mneg : rc.mpos;
+      atomicOr(&mask[feat_from(m, p.stm)], bit);
+      atomicOr(&mask[feat_to(m, p.stm)], bit);
+      atomicOr(&mask[feat_piece(m)], bit);
+      const int fc = feat_capt(m);
+      if (fc >= 0) atomicOr(&mask[fc], bit);
+    }
   }
   __syncthreads();
   constexpr int CH = NINP / 8;  // 16-B chunks per X row
@@ -128,14 +132,16 @@ __device__ __forceinline__ uint2 pack4(const u16 *v) {
 
 // E_SPLITK: the four wavefronts of a block take four K slices of one tile and
 // add their partial tiles in LDS; the block then adds the sum into C (zeroed
-// before) with FP32 atomics, ksplit/4 blocks per tile.
+// before) with FP32 atomics, ksplit/4 blocks per tile.  E_HEAD: one block per
+// tile, its four wavefronts take four K slices and, after the LDS sum, one
+// row group (accumulator registers 4w..4w+3) each for the epilogue.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int lda, const u16 *__restrict__ Bt,
                                                  int ldb, int M, int N, int K, int ksplit, Epi ep) {
   const int lane = threadIdx.x & 63;
   const int tn_count = N / 32;
   int tile, ks;
-  if (EPI == E_SPLITK) {
+  if (EPI == E_SPLITK || EPI == E_HEAD) {
     const int bpt = ksplit / 4;
     tile = blockIdx.x / bpt;
     ks = (blockIdx.x % bpt) * 4 + (threadIdx.x >> 6);
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
     ks = 0;
   }
   const int tm = tile / tn_count, tn = tile % tn_count;
-  if (EPI != E_SPLITK && tm * 32 >= M) return;
+  if (EPI != E_SPLITK && EPI != E_HEAD && tm * 32 >= M) return;
   // K slices of whole 16-steps; the last ones may be short or empty
   const int kc = ((K / 16 + ksplit - 1) / ksplit) * 16, k0 = ks * kc;
   const int kn = K - k0 < kc ? K - k0 : kc;
@@ -211,39 +217,47 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
     cs += __shfl_xor(cs, 32, 64);
     if (lane < 32) atomicAdd(ep.grad + I_B1 + col, cs);
   } else if (EPI == E_HEAD) {
-    // N = 32: the tile holds all of layer 2.  h2 = relu(z2 + b2), s = w3.h2 + b3
-    // (sum over the 32 lanes of a half), then for the 16 pairs of the tile the
-    // loss softplus(s_neg - s_pos + o/PLAIN_PER_LOGIT), dz2 and the b2, W3,
-    // b3 gradients and the loss summed per wavefront
+    // N = 32: the tile holds all of layer 2.  The K slices' partial tiles
+    // are summed in LDS; wavefront w then takes rows 8w + 4*half + j (j < 4,
+    // registers 4w + j): h2 = relu(z2 + b2), s = w3.h2 + b3 (sum over the 32
+    // lanes of a half), and for its 2 pairs per half the loss
+    // softplus(s_neg - s_pos + o/PLAIN_PER_LOGIT), dz2 and the b2, W3, b3
+    // gradients and the loss summed per wavefront
+    __shared__ float red[4][16][64];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 16; i++) red[w][i][lane] = acc[i];
+    __syncthreads();
     const int o = lane & 31;
     const float b2o = ep.prm[I_B2 + o], w3o = ep.prm[I_W3 + o], b3 = ep.prm[I_B3];
-    float h[16], sc[16];
+    float h[4], sc[4];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-      const float z = acc[i] + b2o;
-      h[i] = z > 0.f ? z : 0.f;
-      float t = w3o * h[i];
+    for (int j = 0; j < 4; j++) {
+      const int i = 4 * w + j;
+      const float z = red[0][i][lane] + red[1][i][lane] + red[2][i][lane] + red[3][i][lane] + b2o;
+      h[j] = z > 0.f ? z : 0.f;
+      float t = w3o * h[j];
 #pragma unroll
       for (int x = 16; x > 0; x >>= 1) t += __shfl_xor(t, x, 64);
-      sc[i] = t + b3;
+      sc[j] = t + b3;
     }
     const float invP = 1.f / (float)ep.P;
     float gb2 = 0.f, gw3 = 0.f, gb3 = 0.f, gl = 0.f;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int i = 2 * q;  // rows i, i+1 of this lane: an even row (cutoff move) and its pair
-      const int row = tm * 32 + 8 * (i >> 2) + 4 * half + (i & 3);
-      const float d = sc[i + 1] - sc[i] + (float)pair_offset(ep.recs[ep.idx[row >> 1]].depth) / (float)PLAIN_PER_LOGIT;
+    for (int q = 0; q < 2; q++) {
+      const int j = 2 * q;  // rows j, j+1 of this lane's group: an even row (cutoff move) and its pair
+      const int row = tm * 32 + 8 * w + 4 * half + j;
+      const float d = sc[j + 1] - sc[j] + (float)pair_offset(ep.recs[ep.idx[row >> 1]].depth) / (float)PLAIN_PER_LOGIT;
       const float sig = 1.f / (1.f + expf(-d));
       const float l = d > 0.f ? d + log1pf(expf(-d)) : log1pf(expf(d));
       const float ds0 = -sig * invP, ds1 = sig * invP;
-      const float dz0 = h[i] > 0.f ? ds0 * w3o : 0.f, dz1 = h[i + 1] > 0.f ? ds1 * w3o : 0.f;
+      const float dz0 = h[j] > 0.f ? ds0 * w3o : 0.f, dz1 = h[j + 1] > 0.f ? ds1 * w3o : 0.f;
       const u16 d0 = f2bf(dz0), d1 = f2bf(dz1);
       ep.H[(size_t)row * NH2 + o] = d0;
       ep.H[(size_t)(row + 1) * NH2 + o] = d1;
       *(u32 *)(ep.Ht + (size_t)o * M + row) = (u32)d0 | ((u32)d1 << 16);
       gb2 += dz0 + dz1;
-      gw3 += ds0 * h[i] + ds1 * h[i + 1];
+      gw3 += ds0 * h[j] + ds1 * h[j + 1];
       if (o == 0) {
         gb3 += ds0 + ds1;
         gl += l * invP;
@@ -369,6 +383,9 @@ int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, int M, int N, int K, int
   if (EPI == E_SPLITK) {
     ksplit = 4;
     while (tiles * ksplit < 2048 && K / (2 * ksplit) >= 128) ksplit *= 2;
+  } else if (EPI == E_HEAD) {
+    if (N != 32) return zr::fail(ZETA_EINVAL, "mlp gemm: the head product needs N = 32");
+    ksplit = 4;
   }
   if (M % 32 || N % 32 || K % 16) return zr::fail(ZETA_EINVAL, "mlp gemm: shape %d x %d x %d", M, N, K);
   ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt<EPI>, nb(tiles * ksplit * 64, 256), 256, 0, A, lda, Bt, ldb, M, N, K,

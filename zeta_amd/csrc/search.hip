@@ -817,6 +817,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
       // the best score any worker has found for a root move in this
       // iteration (refreshed at the poll); -INF until one has
       s32 ra = -INF;
+      {
+        // a worker joining the iteration starts from the bound the root's
+        // other workers have published
+        u64 rb = 0;
+        if (lane == 0) rb = ag_load(&a.rbound[r]);
+        rb = uf(rb);
+        if ((rb >> 32) == (ticket & 0xFFFFFFFFULL) && (int)((rb >> 16) & 0xFFFF) == id)
+          ra = (s32)(rb & 0xFFFF) - 32768;
+      }
       Node n;
       {
         u64 q[8];
@@ -1144,16 +1153,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
             u32 b = 0;
             if (lane == 0) b = __atomic_load_n(&a.busy[bidx], __ATOMIC_RELAXED);
             b = uf(b);
-            // batch searches defer the first child too: a worker that finds
-            // it busy moves on to the siblings and comes back to it in the
-            // second pass, which keeps helpers off the path every worker
-            // starts on (10-15 % shorter time to depth on the 1024-root
-            // depth-6 batch).  Served requests, deep and with many workers
-            // on one root, defer only later siblings: there the first-child
-            // deferral sends helpers into siblings searched with poor bounds
-            // (a depth 4-9 stream completed 43 instead of 47 of 48 requests
-            // within 120 s with it)
-            if (n.iter == 1 && b && (!serving || n.searched > 0)) {
+            // batch searches defer a busy first child too, from ply 3 down:
+            // a worker that finds it busy moves on to the siblings and comes
+            // back to it in the second pass, which keeps helpers off the
+            // path every worker starts on.  At the root and ply 2 the first
+            // child is searched by every worker that reaches it: a sibling
+            // searched before the first child has set the root's bound runs
+            // with the full window (with 128 workers per root the depth-7
+            // 1024-root batch took 44-45 s this way against 64-67 s with
+            // the deferral at every ply, and 51 s from ply 4 down).  Served
+            // requests defer only later siblings (a depth 4-9 stream
+            // completed 43 instead of 47 of 48 requests within 120 s with
+            // the first-child deferral at every ply)
+            if (n.iter == 1 && b && (n.searched > 0 || (!serving && sd >= 3))) {
               if (lane == 0) {
                 mscore[k] = (int16_t)sk;
                 mmove[k] = m | DEFERRED;

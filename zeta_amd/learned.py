@@ -25,22 +25,27 @@ from . import (MLP_NPARAMS, mlp_get, mlp_grad, mlp_init, mlp_set, mlp_train, pla
                records_count, records_get, records_reserve, search, search_counters)
 
 
-def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, chunk=4096, progress=None):
+def generate_records(seed=4, n=20000, depth=5, workers=16, capacity=3_000_000, chunk=4096, progress=None,
+                     log_shift=None):
     """Log training pairs from plain searches of n playout positions (20-60 plies).
 
     Cut nodes are sampled by hash (1 in 2^shift) so that the pairs spread over
     all n positions instead of filling the buffer with the first ones; the
-    rate is set from a probe of the first 256 positions."""
+    rate is log_shift, or set from a probe of the first 256 positions."""
     records_reserve(capacity)
     pos = playouts(seed, n, 20, 60)
     t = time.perf_counter()
     records_clear()
-    probe = min(256, n)
-    search(pos[:probe], depth, workers=workers, tt_log2=26, clear_tt=True, log_records=True, log_shift=0)
-    per_pos = max(records_count(), 1) / probe
-    shift = 0
-    while shift < 20 and per_pos * n / (1 << shift) > 0.8 * capacity:
-        shift += 1
+    per_pos = None
+    if log_shift is None:
+        probe = min(256, n)
+        search(pos[:probe], depth, workers=workers, tt_log2=26, clear_tt=True, log_records=True, log_shift=0)
+        per_pos = max(records_count(), 1) / probe
+        shift = 0
+        while shift < 20 and per_pos * n / (1 << shift) > 0.8 * capacity:
+            shift += 1
+    else:
+        shift = int(log_shift)
     records_clear()
     for i in range(0, n, chunk):
         search(pos[i:i + chunk], depth, workers=workers, tt_log2=26, clear_tt=(i == 0), log_records=True,

@@ -25,9 +25,10 @@ def make_stream(rate, n, seed=5, dmin=4, dmax=9):
 
 
 def run_stream(pos, depths, arrivals, learned=False, slots=256, workers=16, tt_log2=26, mem_budget=0,
-               mlp_min_depth=2, progress=None, max_seconds=None):
+               mlp_min_depth=2, progress=None, max_seconds=None, grid=0):
     """Run the stream; with max_seconds, requests still running (or not yet
-    admitted) when it expires are cancelled and reported as such."""
+    admitted) when it expires are cancelled and reported as such.  grid:
+    persistent wavefronts of the service (0 = as many as fit)."""
     n = len(pos)
     finish = np.full(n, np.nan)
     res = {}
@@ -35,7 +36,7 @@ def run_stream(pos, depths, arrivals, learned=False, slots=256, workers=16, tt_l
     backlog = collections.deque()
     i = 0
     with Server(slots=slots, workers=workers, tt_log2=tt_log2, use_mlp=learned, mlp_min_depth=mlp_min_depth,
-                mem_budget=mem_budget) as sv:
+                mem_budget=mem_budget, grid=grid) as sv:
         t0 = time.perf_counter()
         last_report = t0
         expired = False
