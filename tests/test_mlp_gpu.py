@@ -39,9 +39,10 @@ def test_params_roundtrip(zeta):
     assert np.array_equal(zeta.mlp_get(), w)
 
 
-@pytest.mark.parametrize("npairs", [64, 2048])
+@pytest.mark.parametrize("npairs", [48, 64, 2048])
 def test_gradient_matches_oracle(zeta, oracle, npairs):
-    # 64 pairs: each K-split product in one block per tile; 2048 pairs (the
+    # 48 pairs: a row-tile count (3) that leaves waves of a block idle; 64
+    # pairs: each K-split product in one block per tile; 2048 pairs (the
     # training batch): several blocks per tile adding into the gradient
     pairs = _pairs(oracle, npairs, 1)
     w = np.random.default_rng(5).normal(0, 0.08, M.NPARAMS).astype(np.float32)

@@ -145,21 +145,65 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
     const int bpt = ksplit / 4;
     tile = blockIdx.x / bpt;
     ks = (blockIdx.x % bpt) * 4 + (threadIdx.x >> 6);
+  } else if (EPI == E_L1BACK) {
+    // the four wavefronts of a block share a column tile (their b1 column
+    // sums meet in LDS)
+    tile = ((int)(blockIdx.x / tn_count) * 4 + (int)(threadIdx.x >> 6)) * tn_count + (int)(blockIdx.x % tn_count);
+    ks = 0;
   } else {
     tile = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     ks = 0;
   }
   const int tm = tile / tn_count, tn = tile % tn_count;
-  if (EPI != E_SPLITK && EPI != E_HEAD && tm * 32 >= M) return;
+  if (EPI != E_SPLITK && EPI != E_HEAD && EPI != E_L1BACK && tm * 32 >= M) return;
   // K slices of whole 16-steps; the last ones may be short or empty
+  const bool live = tm * 32 < M;  // E_L1BACK: false for the waves past M
   const int kc = ((K / 16 + ksplit - 1) / ksplit) * 16, k0 = ks * kc;
-  const int kn = K - k0 < kc ? K - k0 : kc;
+  const int kn = !live ? 0 : (K - k0 < kc ? K - k0 : kc);
   const int r = tm * 32 + (lane & 31), c = tn * 32 + (lane & 31);
   const int kofs = (lane >> 5) * 8;
   const u16 *pa = A + (size_t)r * lda + kofs + k0;
   const u16 *pb = Bt + (size_t)c * ldb + kofs + k0;
+  // E_HEAD: the plain score differences of this lane's two pairs, loaded
+  // (index, then record) while the products run
+  u32 hdw[2] = {0u, 0u};
+  if (EPI == E_HEAD) {
+    const int rb = tm * 32 + 8 * (threadIdx.x >> 6) + 4 * (lane >> 5);
+    hdw[0] = ep.recs[ep.idx[rb >> 1]].depth;
+    hdw[1] = ep.recs[ep.idx[(rb + 2) >> 1]].depth;
+  }
   f32x16 acc = {};
-  for (int k = 0; k < kn; k += 16) {
+  // four 16-steps per trip, the next trip's operands loaded before this
+  // trip's MFMAs: the loads of one trip overlap the products of the last
+  // instead of each product waiting for its own load
+  int k = 0;
+  if (kn >= 64) {
+    bf16x8 a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      a[j] = *(const bf16x8 *)(pa + 16 * j);
+      b[j] = *(const bf16x8 *)(pb + 16 * j);
+    }
+    for (; k + 128 <= kn; k += 64) {
+      bf16x8 na[4], nb[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        na[j] = *(const bf16x8 *)(pa + k + 64 + 16 * j);
+        nb[j] = *(const bf16x8 *)(pb + k + 64 + 16 * j);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[j], acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        a[j] = na[j];
+        b[j] = nb[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[j], acc, 0, 0, 0);
+    k += 64;
+  }
+  for (; k < kn; k += 16) {
     const bf16x8 a = *(const bf16x8 *)(pa + k);
     const bf16x8 b = *(const bf16x8 *)(pb + k);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
@@ -202,20 +246,26 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
   } else if (EPI == E_L1BACK) {
     // dz1 = dh1 * relu'(h1): BF16 transposed for the W1 gradient; b1 gradient
     float cs = 0.f;
+    if (live) {
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const int rb = tm * 32 + 8 * g + 4 * half;
-      u16 db[4];
+      for (int g = 0; g < 4; g++) {
+        const int rb = tm * 32 + 8 * g + 4 * half;
+        u16 db[4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const float dz = ep.Zf[(size_t)(rb + j) * N + col] > 0.f ? acc[4 * g + j] : 0.f;
-        cs += dz;
-        db[j] = f2bf(dz);
+        for (int j = 0; j < 4; j++) {
+          const float dz = ep.Zf[(size_t)(rb + j) * N + col] > 0.f ? acc[4 * g + j] : 0.f;
+          cs += dz;
+          db[j] = f2bf(dz);
+        }
+        *(uint2 *)(ep.Ht + (size_t)col * M + rb) = pack4(db);
       }
-      *(uint2 *)(ep.Ht + (size_t)col * M + rb) = pack4(db);
     }
     cs += __shfl_xor(cs, 32, 64);
-    if (lane < 32) atomicAdd(ep.grad + I_B1 + col, cs);
+    __shared__ float cls[4][32];
+    if (lane < 32) cls[threadIdx.x >> 6][lane] = cs;
+    __syncthreads();
+    if (threadIdx.x < 32)
+      atomicAdd(ep.grad + I_B1 + col, cls[0][lane] + cls[1][lane] + cls[2][lane] + cls[3][lane]);
   } else if (EPI == E_HEAD) {
     // N = 32: the tile holds all of layer 2.  The K slices' partial tiles
     // are summed in LDS; wavefront w then takes rows 8w + 4*half + j (j < 4,
@@ -247,7 +297,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
     for (int q = 0; q < 2; q++) {
       const int j = 2 * q;  // rows j, j+1 of this lane's group: an even row (cutoff move) and its pair
       const int row = tm * 32 + 8 * w + 4 * half + j;
-      const float d = sc[j + 1] - sc[j] + (float)pair_offset(ep.recs[ep.idx[row >> 1]].depth) / (float)PLAIN_PER_LOGIT;
+      const float d = sc[j + 1] - sc[j] + (float)pair_offset(hdw[q]) / (float)PLAIN_PER_LOGIT;
       const float sig = 1.f / (1.f + expf(-d));
       const float l = d > 0.f ? d + log1pf(expf(-d)) : log1pf(expf(d));
       const float ds0 = -sig * invP, ds1 = sig * invP;
@@ -267,13 +317,24 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
     gw3 += __shfl_xor(gw3, 32, 64);
     gb3 += __shfl_xor(gb3, 32, 64);
     gl += __shfl_xor(gl, 32, 64);
+    // the four wavefronts' sums meet in LDS: one atomic per value and block
+    // (every block adds into the same 66 words)
+    __shared__ float hs[4][2 * NH2 + 2];
     if (lane < 32) {
-      atomicAdd(ep.grad + I_B2 + o, gb2);
-      atomicAdd(ep.grad + I_W3 + o, gw3);
+      hs[w][o] = gb2;
+      hs[w][NH2 + o] = gw3;
     }
     if (lane == 0) {
-      atomicAdd(ep.grad + I_B3, gb3);
-      atomicAdd(ep.loss, gl);
+      hs[w][2 * NH2] = gb3;
+      hs[w][2 * NH2 + 1] = gl;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 2 * NH2 + 2) {
+      const float v = hs[0][t] + hs[1][t] + hs[2][t] + hs[3][t];
+      float *dst = t < NH2 ? ep.grad + I_B2 + t
+                           : (t < 2 * NH2 ? ep.grad + I_W3 + (t - NH2) : (t == 2 * NH2 ? ep.grad + I_B3 : ep.loss));
+      atomicAdd(dst, v);
     }
   }
 }
@@ -382,14 +443,16 @@ int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, int M, int N, int K, int
   int ksplit = 1;
   if (EPI == E_SPLITK) {
     ksplit = 4;
-    while (tiles * ksplit < 2048 && K / (2 * ksplit) >= 128) ksplit *= 2;
+    while (tiles * ksplit < 1024 && K / (2 * ksplit) >= 128) ksplit *= 2;
   } else if (EPI == E_HEAD) {
     if (N != 32) return zr::fail(ZETA_EINVAL, "mlp gemm: the head product needs N = 32");
     ksplit = 4;
   }
   if (M % 32 || N % 32 || K % 16) return zr::fail(ZETA_EINVAL, "mlp gemm: shape %d x %d x %d", M, N, K);
-  ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt<EPI>, nb(tiles * ksplit * 64, 256), 256, 0, A, lda, Bt, ldb, M, N, K,
-          ksplit, ep);
+  // E_L1BACK: blocks of four row tiles in one column tile (the last block's
+  // extra waves, past M, only join its barrier)
+  const size_t blocks = EPI == E_L1BACK ? (size_t)((M / 32 + 3) / 4) * (N / 32) : nb(tiles * ksplit * 64, 256);
+  ZLAUNCH(kid, (u64)M * N * K * 2, k_gemm_nt<EPI>, blocks, 256, 0, A, lda, Bt, ldb, M, N, K, ksplit, ep);
   return ZETA_OK;
 }
 
