@@ -130,12 +130,12 @@ def mlp_leg(zeta_amd, steps=400, pairs_per_step=2048):
                          "note": "dense FLOP over the summed time of the forward and backward kernels"}}
 
 
-def c5_leg(zeta_amd, rate=2000.0, n=48, max_seconds=60.0, workers=64, grid=1024):
+def c5_leg(zeta_amd, rate=2000.0, n=48, max_seconds=60.0, workers=128, grid=0):
     """Serving: a Poisson stream (seed 5) of n requests at `rate`, depths
     uniform in 4-9, plain then learned ordering, each stream capped at
     max_seconds (requests still running then are cancelled and reported);
     up to `workers` wavefronts per request on a persistent grid of `grid`
-    wavefronts (one per SIMD)."""
+    wavefronts (0 = as many as fit)."""
     from zeta_amd import learned as L
     from zeta_amd import serving as S
     L.load(NET_PATH)

@@ -201,7 +201,7 @@ int zeta_search_counters(uint64_t *out, int n);
  * watchdog_ms the grid cancels its requests and exits. */
 typedef struct {
   int slots;            /* request slots (requests in flight), 1..4096 */
-  int workers;          /* default and maximum workers per request, 1..64 */
+  int workers;          /* default and maximum workers per request, 1..255 */
   int tt_log2;          /* shared TT size 2^tt_log2 x 16 B; -1 = zeta_config.tt_log2; 0 = none */
   int use_mlp;          /* requests with ZETA_REQ_LEARNED use the loaded network */
   int mlp_min_depth;

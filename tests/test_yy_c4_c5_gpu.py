@@ -65,8 +65,7 @@ def _c5_streams(zeta):
         runs = {}
         for mode in ("plain", "learned"):
             out, lat, wall, wn = S.run_stream(pos, depths, arrivals, learned=(mode == "learned"), slots=64,
-                                              workers=64, tt_log2=24, mlp_min_depth=3, max_seconds=40.0,
-                                              grid=1024)
+                                              workers=128, tt_log2=24, mlp_min_depth=3, max_seconds=40.0)
             runs[mode] = out
             print(json.dumps({mode: S.summarize(out, lat, wall, depths, wn)}), flush=True)
         _cache["c5"] = (pos, depths, runs)

@@ -1703,8 +1703,8 @@ extern "C" int zeta_serve_start(const zeta_serve_opts *o) {
   if (r) return r;
   if (!o) return zr::fail(ZETA_EINVAL, "zeta_serve_start: null options");
   if (o->slots < 1 || o->slots > 4096) return zr::fail(ZETA_EINVAL, "zeta_serve_start: slots %d not in 1..4096", o->slots);
-  if (o->workers < 1 || o->workers > 64)
-    return zr::fail(ZETA_EINVAL, "zeta_serve_start: workers %d not in 1..64", o->workers);
+  if (o->workers < 1 || o->workers > 255)  // 8 bits of the slot word
+    return zr::fail(ZETA_EINVAL, "zeta_serve_start: workers %d not in 1..255", o->workers);
   const int tt_log2 = o->tt_log2 < 0 ? zr::ctx().cfg.tt_log2 : o->tt_log2;
   if (tt_log2 > 34) return zr::fail(ZETA_EINVAL, "zeta_serve_start: tt_log2 %d too large", tt_log2);
   const zm::Weights *w = nullptr;
