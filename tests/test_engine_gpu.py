@@ -42,7 +42,7 @@ def test_guessconfig_writes_and_reads_back_the_launch_knobs(tmp_path):
     p = run("", args=("--guessconfig",), config=cfg, timeout=200)
     assert p.returncode == 0
     vals = {k: int(v) for k, v in re.findall(r"^(\w+): (-?\d+);", open(cfg).read(), re.M)}
-    assert vals["threadsX"] in (1, 4, 16, 64) and vals["threadsY"] == 1
+    assert vals["threadsX"] in (1, 4, 16, 64, 128) and vals["threadsY"] == 1
     assert vals["split_depth"] in (1, 2, 3) and vals["waves_per_simd"] in (1, 2)
     assert vals["nodes_per_second"] > 0 and vals["tt1_memory"] == 1024
     assert ("#> config read back: threadsX %d, split_depth %d, waves_per_simd %d"

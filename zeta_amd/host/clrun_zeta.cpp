@@ -49,7 +49,7 @@ int tt_log2_of(uint64_t mb) {
 
 int workers_of() {
   const uint64_t w = totalWorkUnits ? totalWorkUnits : 1;
-  return w > 64 ? 64 : (int)w;
+  return w > 1024 ? 1024 : (int)w;  // zeta_search_opts.workers' range
 }
 
 }  // namespace

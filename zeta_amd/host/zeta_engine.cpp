@@ -638,7 +638,8 @@ bool guess_config(bool extended) {
   };
   int best_w = 1, best_sd = 2, best_wps = 0;
   double best_t = 0, best_nps = 0;
-  const std::vector<int> wc = extended ? std::vector<int>{1, 2, 4, 8, 16, 32, 64} : std::vector<int>{1, 4, 16, 64};
+  const std::vector<int> wc =
+      extended ? std::vector<int>{1, 2, 4, 8, 16, 32, 64, 128, 256} : std::vector<int>{1, 4, 16, 64, 128};
   for (int w : wc) {
     double t, nps;
     if (!single(w, best_sd, t, nps)) return fail("workers");
