@@ -333,28 +333,46 @@ __device__ __forceinline__ void busy_release(const SearchArgs &a, u64 hash, u64 
 
 // next root for this worker: the queue first, then the unfinished root with
 // the fewest workers (ABDADA helper), at most a.workers per root
-__device__ int pick_root(const SearchArgs &a, int hstart) {
-  int r = atomicAdd(a.queue, 1);
+// A worker takes the next unstarted root; once all have started it joins,
+// as an ABDADA helper below the per-root limit, the unfinished root with the
+// fewest completed iterations (the most work left: an iteration costs
+// several times the one before it), then the fewest workers.  The wave's
+// lanes scan the roots together and agree on the minimum.
+__device__ int pick_root(const SearchArgs &a, int hstart, int lane) {
+  int r = 0;
+  if (lane == 0) r = atomicAdd(a.queue, 1);
+  r = uf(r);
   if (r < a.nroots) {
-    atomicAdd(&a.nworkers[r], 1);
+    if (lane == 0) atomicAdd(&a.nworkers[r], 1);
     return r;
   }
   if (a.workers <= 1) return -1;
   for (int attempt = 0; attempt < 8; attempt++) {
-    int best = -1, bw = a.workers;
-    for (int k = 0; k < a.nroots; k++) {
+    u64 best = ~0ULL;  // (iterations done << 48) | (workers << 32) | root
+    for (int k = lane; k < a.nroots; k += 64) {
       const int rr = (hstart + k) % a.nroots;
       if (__atomic_load_n(&a.done[rr], __ATOMIC_RELAXED)) continue;
       const int w = __atomic_load_n(&a.nworkers[rr], __ATOMIC_RELAXED);
-      if (w < bw) {
-        bw = w;
-        best = rr;
-        if (w <= 1) break;
-      }
+      if (w >= a.workers) continue;
+      const int dd = __atomic_load_n(&a.depth_done[rr], __ATOMIC_RELAXED);
+      const u64 key = ((u64)(dd < 0 ? 0 : (dd > 255 ? 255 : dd)) << 48) | ((u64)(u32)w << 32) | (u32)rr;
+      best = key < best ? key : best;
     }
-    if (best < 0) return -1;
-    if (atomicAdd(&a.nworkers[best], 1) < a.workers) return best;
-    atomicSub(&a.nworkers[best], 1);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const u64 other = ((u64)(u32)__shfl_xor((int)(u32)(best >> 32), o, 64) << 32) |
+                        (u32)__shfl_xor((int)(u32)best, o, 64);
+      best = other < best ? other : best;
+    }
+    best = uf(best);
+    if (best == ~0ULL) return -1;
+    const int pick = (int)(u32)best;
+    int got = 0;
+    if (lane == 0) {
+      got = atomicAdd(&a.nworkers[pick], 1) < a.workers ? 1 : 0;
+      if (!got) atomicSub(&a.nworkers[pick], 1);
+    }
+    if (uf(got)) return pick;
   }
   return -1;
 }
@@ -767,9 +785,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 8))) void
     if (serving) {
       r = serve_pick(a, lane, ticket);
     } else {
-      int rr = -1;
-      if (lane == 0) rr = sys_load32(a.habort) ? -1 : pick_root(a, hstart);
-      r = uf(rr);
+      u32 ab = 0;
+      if (lane == 0) ab = sys_load32(a.habort);
+      r = uf(ab) ? -1 : pick_root(a, hstart, lane);
     }
     if (r < 0) return;
     int depth_r = a.depth;
