@@ -126,7 +126,9 @@ def main():
     ap.add_argument("--dmin", type=int, default=4)
     ap.add_argument("--dmax", type=int, default=9)
     ap.add_argument("--slots", type=int, default=256)
-    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--workers", type=int, default=128)
+    ap.add_argument("--grid", type=int, default=0, help="persistent wavefronts (0 = as many as fit)")
+    ap.add_argument("--mlp-min-depth", type=int, default=3)
     ap.add_argument("--max-seconds", type=float, default=None,
                     help="cancel what is still running after this long (per stream and mode)")
     ap.add_argument("--net", default=None, help="trained network (npz); without it the learned run uses a "
@@ -140,7 +142,9 @@ def main():
     prog = lambda d: print(json.dumps(d), flush=True)  # noqa: E731
     for r in [float(x) for x in args.rates.split(",")]:
         rep = compare(r, args.requests, dmin=args.dmin, dmax=args.dmax, slots=args.slots, workers=args.workers,
-                      progress=prog, max_seconds=args.max_seconds)
+                      progress=prog, max_seconds=args.max_seconds, grid=args.grid, mlp_min_depth=args.mlp_min_depth)
+        for mode in ("plain", "learned"):
+            rep[mode].pop("by_depth", None)
         print(json.dumps(rep), flush=True)
 
 
