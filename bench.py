@@ -41,8 +41,8 @@ HBM_BYTES_PER_POSITION = 12.85
 # over the depth-6 search leg (profiles/search_c3_pmc.md).  A CU's arbiter
 # issues at most one scalar instruction per cycle (each SIMD's waves get one
 # SALU slot every 4 cycles), so the SALU issue peak is 256 CU x 2.4 GHz.
-VALU_WAVE_INSTS_PER_NODE = 449.2
-SALU_INSTS_PER_NODE = 867.6
+VALU_WAVE_INSTS_PER_NODE = 459.8
+SALU_INSTS_PER_NODE = 870.1
 SALU_PEAK = 256 * 2.4e9
 # the ordering network: 908 -> 256 -> 32 -> 1, dense FLOP of one scored move
 # (2 x (908 x 256 + 256 x 32 + 32)); a training pair scores two moves and its
