@@ -114,9 +114,9 @@ __global__ __launch_bounds__(256) void k_features(const PairRec *__restrict__ re
 // 32x32 tile of C (and K slice), with the consumer of each product fused in
 // as its epilogue.  Accumulator layout of v_mfma_f32_32x32x16_bf16: register
 // i of lane l holds C[8*(i/4) + 4*(l/32) + i%4][l%32].
-enum { E_STORE, E_SPLITK, E_L1ACT, E_HEAD, E_L1BACK };
+enum { E_SPLITK, E_L1ACT, E_HEAD, E_L1BACK };
 struct Epi {
-  float *C;                   // E_STORE / E_SPLITK output, ldc = N
+  float *C;                   // E_SPLITK output, ldc = N
   const float *prm;           // master weights (biases, head)
   float *Zf;                  // E_L1ACT: h1 (FP32) out;  E_L1BACK: h1 in
   u16 *H, *Ht;                // E_L1ACT: h1 BF16 [M][N] and [N][M];  E_L1BACK: Ht = dz1 [N][M];  E_HEAD: dz2, dz2t
@@ -209,10 +209,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
   const int half = lane >> 5, col = tn * 32 + (lane & 31);
-  if (EPI == E_STORE) {
-#pragma unroll
-    for (int i = 0; i < 16; i++) ep.C[(size_t)(tm * 32 + 8 * (i >> 2) + 4 * half + (i & 3)) * N + col] = acc[i];
-  } else if (EPI == E_SPLITK) {
+  if (EPI == E_SPLITK) {
     __shared__ float red[3][16][64];
     const int w = threadIdx.x >> 6;
     if (w) {
