@@ -83,3 +83,24 @@ def test_bf16_variant_close_to_exact():
     l16, g16 = M.forward_backward(w.astype(np.float32), pairs, bf16_points=True)
     assert abs(l64 - l16) < 2e-2 * max(1.0, abs(l64))
     assert np.linalg.norm(g16 - g64) < 5e-2 * np.linalg.norm(g64)
+
+
+def test_committed_network_checkpoint():
+    """The committed ordering network (zeta_amd/order_net.npz, written by
+    zeta_amd.learned.save): BF16 W1 and W2, FP32 biases and head, loadable
+    without pickles, NPARAMS parameters in the C-ABI layout; the oracle
+    scores moves with it to finite values."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "zeta_amd", "order_net.npz")
+    with np.load(path, allow_pickle=False) as z:
+        w1, w2, rest = z["w1_bf16"], z["w2_bf16"], z["rest_f32"]
+    assert w1.dtype == np.uint16 and w1.size == M.NIN * M.NH1
+    assert w2.dtype == np.uint16 and w2.size == M.NH1 * M.NH2
+    assert rest.dtype == np.float32 and rest.size == M.NH1 + M.NH2 + M.NH2 + 1
+    bf = lambda a: (a.astype(np.uint32) << 16).view(np.float32)  # noqa: E731
+    params = np.concatenate([bf(w1), rest[:M.NH1], bf(w2), rest[M.NH1:]])
+    assert params.size == M.NPARAMS and np.isfinite(params).all()
+    p = R.setboard(R.STARTPOS)
+    moves = [R.nth_legal(p, i) for i in range(R.count_legal(p))]
+    s = M.scores(params, p["b"][0, :4], 0, moves)
+    assert len(s) == 20 and np.isfinite(s).all()
