@@ -108,3 +108,22 @@ def test_learned_ordering_keeps_scores_and_logging(zeta, oracle):
         assert int(r["mpos"]) != int(r["mneg"])
         assert int(r["mneg"]) in legal or ((int(r["mneg"]) >> 19) & 7) in (1, 3)  # e.p./castle need flags
     zeta.records_clear()
+
+
+def test_working_copies_follow_training(zeta, oracle):
+    # the Adam kernel also writes the BF16 copies the products read: after
+    # some training steps the gradient the GPU computes must still be the
+    # oracle's for the parameters the training left
+    pairs = _pairs(oracle, 512, 3)
+    zeta.records_put(pairs)
+    zeta.mlp_init(9)
+    zeta.mlp_train(5, pairs_per_step=256, lr=1e-2, seed=5)
+    w = zeta.mlp_get()
+    check = pairs[:64]
+    loss, g = zeta.mlp_grad(check)
+    lref, gref = M.forward_backward(w, check, bf16_points=True)
+    assert abs(loss - lref) <= 1e-3 * max(1.0, abs(lref))
+    parts = np.split(np.arange(M.NPARAMS), np.cumsum([908 * 256, 256, 256 * 32, 32, 32]))
+    for name, ix in zip(("W1", "b1", "W2", "b2", "W3"), parts[:5]):
+        err = np.linalg.norm(g[ix] - gref[ix]) / max(np.linalg.norm(gref[ix]), 1e-12)
+        assert err < 2e-2, name

@@ -336,18 +336,32 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16 *__restrict__ A, int 
   }
 }
 
-// Adam on the internal layout; padded W1 columns stay zero
+// Adam on the internal layout (padded W1 columns stay zero), and in the
+// same pass the BF16 working copies of the updated W1 and W2
 __global__ void k_adam(float *prm, const float *__restrict__ grad, float *m, float *v, int n, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
+                       float b2, float eps, float c1, float c2, u16 *w1t, u16 *w1f, u16 *w2t, u16 *w2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (i < I_B1 && (i % NINP) >= NIN) return;
-  const float g = grad[i];
-  const float mi = b1 * m[i] + (1.f - b1) * g;
-  const float vi = b2 * v[i] + (1.f - b2) * g * g;
-  m[i] = mi;
-  v[i] = vi;
-  prm[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  float p = prm[i];
+  if (!(i < I_B1 && (i % NINP) >= NIN)) {
+    const float g = grad[i];
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    p -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    prm[i] = p;
+  }
+  if (i < I_B1) {
+    const u16 b = f2bf(p);
+    w1t[i] = b;
+    w1f[(i % NINP) * NH1 + i / NINP] = b;
+  } else if (i >= I_W2T && i < I_B2) {
+    const int j = i - I_W2T;
+    const u16 b = f2bf(p);
+    w2t[j] = b;
+    w2[(j % NH1) * NH2 + j / NH1] = b;
+  }
 }
 
 // BF16 working copies of the master weights
@@ -693,8 +707,8 @@ extern "C" int zeta_mlp_train(int steps, int pairs_per_step, float lr, uint64_t 
     g_ms.step++;
     const float c1 = 1.f - powf(b1, (float)g_ms.step), c2 = 1.f - powf(b2, (float)g_ms.step);
     ZLAUNCH(zr::K_MLP_ADAM, NINT, k_adam, nb(NINT, 256), 256, 0, (float *)g_ms.prm.p, (const float *)g_ms.grad.p,
-            (float *)g_ms.m.p, (float *)g_ms.v.p, NINT, lr, b1, b2, 1e-8f, c1, c2);
-    if ((r = refresh())) return r;
+            (float *)g_ms.m.p, (float *)g_ms.v.p, NINT, lr, b1, b2, 1e-8f, c1, c2, (u16 *)g_ms.w1t.p,
+            (u16 *)g_ms.w1f.p, (u16 *)g_ms.w2t.p, (u16 *)g_ms.w2.p);
     if (losses) ZCHECK(hipMemcpyAsync(&lv[s], g_ms.loss.p, 4, hipMemcpyDeviceToHost, st));
   }
   ZCHECK(hipStreamSynchronize(st));
