@@ -60,8 +60,19 @@ constexpr u16 BF_ONE = 0x3F80;
 // layouts (X [B][928] and Xt [928][B]) so that no clearing pass is needed:
 // the block (256 threads) collects a 32-bit row mask per feature in LDS,
 // then stores 16-B chunks of X rows and 64-B row segments of Xt.
-__global__ __launch_bounds__(256) void k_features(const PairRec *__restrict__ recs, const u32 *__restrict__ idx,
-                                                  int B, u16 *X, u16 *Xt) {
+// record index of pair i of a training step: splitmix64 of (seed, i) mod nrec
+__device__ __forceinline__ u32 sample_index(u64 seed, int i, u64 nrec) {
+  u64 z = seed + (u64)(i + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return (u32)(z % nrec);
+}
+
+// nrec > 0: the step's pairs are drawn here (sample_index) and their indices
+// written to idx for the later kernels; nrec = 0: idx holds them already
+__global__ __launch_bounds__(256) void k_features(const PairRec *__restrict__ recs, u32 *__restrict__ idx,
+                                                  int B, u16 *X, u16 *Xt, u64 nrec, u64 seed) {
   __shared__ u32 mask[NINP];
   const int e0 = blockIdx.x * 32;
   for (int f = threadIdx.x; f < NINP; f += blockDim.x) mask[f] = 0;
@@ -70,7 +81,14 @@ __global__ __launch_bounds__(256) void k_features(const PairRec *__restrict__ re
     // eight threads per row, one rank of the board each (the first one also
     // the move's features)
     const int r = threadIdx.x >> 3, rank = threadIdx.x & 7, e = e0 + r;
-    const PairRec rc = recs[idx[e >> 1]];
+    u32 ri;
+    if (nrec) {
+      ri = sample_index(seed, e >> 1, nrec);
+      if (rank == 0 && !(e & 1)) idx[e >> 1] = ri;
+    } else {
+      ri = idx[e >> 1];
+    }
+    const PairRec rc = recs[ri];
     zc::Pos p;
     p.b0 = rc.b0; p.b1 = rc.b1; p.b2 = rc.b2; p.b3 = rc.b3; p.cr = 0; p.stm = rc.stm & 1u;
     const u32 bit = 1u << r;
@@ -399,17 +417,6 @@ __global__ void k_init(float *prm, u64 seed) {
   prm[i] = v;
 }
 
-// random pair indices for a step
-__global__ void k_sample(u32 *idx, int P, u64 nrec, u64 seed) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
-  u64 z = seed + (u64)(i + 1) * 0x9E3779B97F4A7C15ULL;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  z ^= z >> 31;
-  idx[i] = (u32)(z % nrec);
-}
-
 struct MlpState {
   zr::DevBuf prm, grad, m, v, w1t, w1f, w2t, w2;     // weights
   zr::DevBuf X, Xt, Z1, H1, H1t, dZ2, dZ2t, dZ1t, idx, loss, recs_scratch;
@@ -467,16 +474,18 @@ int gemm(const u16 *A, int lda, const u16 *Bt, int ldb, int M, int N, int K, int
   return ZETA_OK;
 }
 
-// forward + backward of one batch of P pairs (indices already in g_ms.idx);
-// gradients into g_ms.grad, mean loss into g_ms.loss
-int fwd_bwd(const PairRec *recs, int P) {
+// forward + backward of one batch of P pairs; gradients into g_ms.grad, mean
+// loss into g_ms.loss.  nrec = 0: the record indices are in g_ms.idx already;
+// nrec > 0: the feature kernel draws them from nrec records with seed
+int fwd_bwd(const PairRec *recs, int P, u64 nrec, u64 seed) {
   hipStream_t st = zr::ctx().stream;
   const int B = 2 * P;
   int r;
   ZCHECK(hipMemsetAsync(g_ms.grad.p, 0, NINT * 4, st));
   ZCHECK(hipMemsetAsync(g_ms.loss.p, 0, 4, st));
-  const u32 *idx = (const u32 *)g_ms.idx.p;
-  ZLAUNCH(zr::K_MLP_FWD, B, k_features, B / 32, 256, 0, recs, idx, B, (u16 *)g_ms.X.p, (u16 *)g_ms.Xt.p);
+  u32 *idx = (u32 *)g_ms.idx.p;
+  ZLAUNCH(zr::K_MLP_FWD, B, k_features, B / 32, 256, 0, recs, idx, B, (u16 *)g_ms.X.p, (u16 *)g_ms.Xt.p, nrec,
+          seed);
   Epi ep = {};
   ep.prm = (const float *)g_ms.prm.p;
   ep.grad = (float *)g_ms.grad.p;
@@ -670,7 +679,7 @@ extern "C" int zeta_mlp_grad(const void *pairs, int npairs, float *loss, float *
   for (int i = 0; i < npairs; i++) idx[i] = (u32)i;
   ZCHECK(hipMemcpyAsync(g_ms.recs_scratch.p, pairs, (size_t)npairs * sizeof(PairRec), hipMemcpyHostToDevice, st));
   ZCHECK(hipMemcpyAsync(g_ms.idx.p, idx.data(), (size_t)npairs * 4, hipMemcpyHostToDevice, st));
-  if ((r = fwd_bwd((const PairRec *)g_ms.recs_scratch.p, npairs))) return r;
+  if ((r = fwd_bwd((const PairRec *)g_ms.recs_scratch.p, npairs, 0, 0))) return r;
   std::vector<float> g(NINT);
   float l = 0.f;
   ZCHECK(hipMemcpyAsync(g.data(), g_ms.grad.p, NINT * 4, hipMemcpyDeviceToHost, st));
@@ -701,9 +710,9 @@ extern "C" int zeta_mlp_train(int steps, int pairs_per_step, float lr, uint64_t 
   std::vector<float> lv(steps > 0 ? steps : 1);
   const float b1 = 0.9f, b2 = 0.999f;
   for (int s = 0; s < steps; s++) {
-    ZLAUNCH(zr::K_MLP_FWD, pairs_per_step, k_sample, nb(pairs_per_step, 256), 256, 0, (u32 *)g_ms.idx.p,
-            pairs_per_step, nrec, (u64)seed + (u64)s * 0xD1B54A32D192ED03ULL);
-    if ((r = fwd_bwd((const PairRec *)zrec::g_recs.p, pairs_per_step))) return r;
+    if ((r = fwd_bwd((const PairRec *)zrec::g_recs.p, pairs_per_step, nrec,
+                     (u64)seed + (u64)s * 0xD1B54A32D192ED03ULL)))
+      return r;
     g_ms.step++;
     const float c1 = 1.f - powf(b1, (float)g_ms.step), c2 = 1.f - powf(b2, (float)g_ms.step);
     ZLAUNCH(zr::K_MLP_ADAM, NINT, k_adam, nb(NINT, 256), 256, 0, (float *)g_ms.prm.p, (const float *)g_ms.grad.p,
