@@ -27,15 +27,15 @@ HBM_PEAK = 8.0e12
 # Work of one k_leaf position (apply the parent's move, analyse, count the
 # legal moves), measured once with rocprofv3 --pmc on this workload and kept
 # as constants (profiles/perft_c2_pmc.md):
-#   SQ_INSTS_VALU = 1,997,643,234 wave instructions per launch of 128,916,248
-#   positions (one position per lane) -> 991.7 VALU lane-instructions per
+#   SQ_INSTS_VALU = 1,993,137,114 wave instructions per launch of 128,916,248
+#   positions (one position per lane) -> 989.5 VALU lane-instructions per
 #   position; each wave64 instruction occupies a 16-lane SIMD for 4 cycles,
 #   so the issue peak is 256 CU x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T/s.
 #   FETCH_SIZE 777,423.5 KiB x 1.99991 + WRITE_SIZE 63,073.4 KiB x 0.99999
 #   per launch, the factors from the library's copy kernel over 1 GiB
 #   (FETCH_SIZE 524,311.3 KiB, WRITE_SIZE 1,048,586.0 KiB) -> 12.85 B of HBM
 #   traffic per position.
-VALU_LANE_INSTS_PER_POSITION = 991.7
+VALU_LANE_INSTS_PER_POSITION = 989.5
 HBM_BYTES_PER_POSITION = 12.85
 # k_search: one node per wavefront; SQ_INSTS_VALU and SQ_INSTS_SALU / nodes
 # over the depth-6 search leg (profiles/search_c3_pmc.md).  A CU's arbiter

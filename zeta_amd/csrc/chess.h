@@ -326,33 +326,26 @@ ZD u64 ep_candidates(const Pos &p, const Legal &L, int &sqep) {
   return c & (bit(sqep + 1) | bit(sqep - 1));
 }
 
-// castling targets of the king (src/zetaperft.cl:1249-1297)
+// castling targets of the king (src/zetaperft.cl:1249-1297).  The rule
+// looks at the eight squares k-4 .. k+3 around the king (the reference's
+// bit() takes them modulo 64), so the rook, occupancy and attack sets are
+// rotated right by k-4: bit i is square k-4+i for every king square, with no
+// branch on where the king stands.  A set bit of `bad` is a failed condition:
+// queen side bits 0-4 (rook on k-4, k-3..k-1 empty, k-2..k not attacked),
+// king side bits 4-7 (k+1, k+2 empty, k..k+2 not attacked, rook on k+3).
+ZD u64 rotr64(u64 x, int n) { return shrm(x, n) | shlm(x, 64 - n); }
 ZD u64 castle_targets(const Pos &p, const Legal &L) {
   const u64 cr = p.cr;
-  const int k = L.ksq;
   if (!(cr & SMCRALL)) return 0;
   const bool okq = p.stm ? ((~cr & SMCRBLACKQ) == SMCRBLACKQ) : ((~cr & SMCRWHITEQ) == SMCRWHITEQ);
   const bool okk = p.stm ? ((~cr & SMCRBLACKK) == SMCRBLACKK) : ((~cr & SMCRWHITEK) == SMCRWHITEK);
   if (!okq && !okk) return 0;
   const u64 ownrooks = L.own & L.rq & ~p.b2;  // rook = 101 (queen has b2 set)
-  if (k >= 4 && k <= 60) {
-    // the squares k-4..k+3 do not wrap: each group is one shifted constant
-    // (a king with a castle right is on e1/e8 unless the FEN was inconsistent)
-    const int q = k - 4;
-    u64 t = sel(okq && (ownrooks & shlm(0x1ULL, q)) && !(L.occ & shlm(0xEULL, q)) && !(L.A & shlm(0x1CULL, q)),
-                shlm(0x4ULL, q));
-    t |= sel(okk && (ownrooks & shlm(0x8ULL, k)) && !(L.occ & shlm(0x6ULL, k)) && !(L.A & shlm(0x7ULL, k)),
-             shlm(0x4ULL, k));
-    return t;
-  }
-  u64 t = 0;
-  t |= sel(okq && (ownrooks & bit(k - 4)) && !(L.occ & (bit(k - 1) | bit(k - 2) | bit(k - 3))) &&
-               !(L.A & (bit(k) | bit(k - 1) | bit(k - 2))),
-           bit(k - 2));
-  t |= sel(okk && (ownrooks & bit(k + 3)) && !(L.occ & (bit(k + 1) | bit(k + 2))) &&
-               !(L.A & (bit(k) | bit(k + 1) | bit(k + 2))),
-           bit(k + 2));
-  return t;
+  const int q = (L.ksq - 4) & 63;
+  const u32 rk = (u32)rotr64(ownrooks, q), oc = (u32)rotr64(L.occ, q), at = (u32)rotr64(L.A, q);
+  const u32 bad = (~rk & 0x81u) | (oc & 0x6Eu) | (at & 0x7Cu);
+  const u32 t = ((okq && !(bad & 0x1Fu)) ? 0x04u : 0u) | ((okk && !(bad & 0xF0u)) ? 0x40u : 0u);
+  return rotr64((u64)t, (64 - q) & 63);  // back: bit 2 -> k-2, bit 6 -> k+2
 }
 
 // Number of legal moves (qs: captures and e.p. only, as the kernels' quiescence).
