@@ -163,6 +163,38 @@ def test_castle_shifted_constants(H):
     assert hits > 100
 
 
+def test_castle_every_king_square_modulo_64(H):
+    # the castling rule on the squares k-4 .. k+3 taken modulo 64 (shift
+    # counts are modulo 64 in the reference, src/zetaperft.cl:1249-1297), for
+    # every king square: rooks on k-4 and k+3, the squares between empty,
+    # nothing attacked -> targets k-2 and k+2; each broken condition removes
+    # exactly its own side
+    sq = lambda k, d: 1 << ((k + d) % 64)
+    cr = 1 << 59  # d8 flagged: the precondition holds, both sides' squares unflagged
+    for stm in (0, 1):
+        for k in range(64):
+            rooks = sq(k, -4) | sq(k, 3)
+            base = dict(occ=rooks | sq(k, 0), rooks=rooks, A=0)
+
+            def targets(occ, rooks, A):
+                args = (0, rooks, 0, rooks, cr, stm, k, occ, rooks | sq(k, 0), A)
+                got = H.hc_castle(*args)
+                assert got == H.hc_castle_per_square(*args), (stm, k)
+                return got
+
+            both, q, kk = sq(k, -2) | sq(k, 2), sq(k, -2), sq(k, 2)
+            assert targets(**base) == both, (stm, k)
+            for d in (-3, -2, -1):  # queen side must be empty
+                assert targets(base["occ"] | sq(k, d), rooks, 0) == kk, (stm, k, d)
+            for d in (1, 2):  # king side must be empty
+                assert targets(base["occ"] | sq(k, d), rooks, 0) == q, (stm, k, d)
+            assert targets(base["occ"], rooks, sq(k, 0)) == 0, (stm, k)  # in check
+            for d, want in ((-2, kk), (-1, kk), (1, q), (2, q), (-3, both), (-4, both), (3, both)):
+                assert targets(base["occ"], rooks, sq(k, d)) == want, (stm, k, d)
+            assert targets(base["occ"] ^ sq(k, -4), rooks ^ sq(k, -4), 0) == kk, (stm, k)
+            assert targets(base["occ"] ^ sq(k, 3), rooks ^ sq(k, 3), 0) == q, (stm, k)
+
+
 def test_single_square_sliders_equal_fills(H):
     # hyperbola attacks of one slider = the occluded fills, random occupancies
     rng = np.random.default_rng(7)
