@@ -67,3 +67,24 @@ extern "C" int fh_suite(int suite, int i, char *out, int cap, int *nmoves) {
   std::memcpy(out, f.c_str(), f.size() + 1);
   return 0;
 }
+
+// The front end's clock: at game ply `ply` and a measured speed of `nps`
+// nodes/s, runs the GUI command `line` (when not empty), then, when
+// elapsed_ms >= 0, the update after an engine move that took elapsed_ms.
+// Writes TimeBase, TimeLeft, TimeInc, MaxTime (ms), MovesLeft, MaxMoves,
+// timemode and MaxNodes to out[8].  Returns 0, or -1 for a line that is not
+// a clock command.
+extern "C" int fh_clock(const char *line, int ply, long long nps, double elapsed_ms, double *out) {
+  PLY = ply;
+  nodes_per_second = nps;
+  if (line && *line) {
+    char cmd[64] = {0};
+    sscanf(line, "%63s", cmd);
+    if (!time_command(cmd, line)) return -1;
+  }
+  if (elapsed_ms >= 0) time_control(elapsed_ms);
+  const double v[8] = {TimeBase, TimeLeft, TimeInc, MaxTime, (double)MovesLeft, (double)MaxMoves, (double)timemode,
+                       (double)MaxNodes};
+  for (int i = 0; i < 8; i++) out[i] = v[i];
+  return 0;
+}

@@ -1,9 +1,12 @@
 """CPU tests of the front end's board I/O (zeta_amd/host/zeta_engine.cpp,
 setboard / createfen after src/io.c:227-532): FEN round trips, and the FEN
 after coordinate moves played on the board, against the standard FEN of the
-same game (castle rights, e.p. target, half-move clock, move number)."""
+same game (castle rights, e.p. target, half-move clock, move number); and of
+its clock (level / st / time and the update after the engine's move,
+src/xboard.c:355-371, 384-455) on worked examples."""
 import ctypes
 import os
+import shutil
 import subprocess
 
 import pytest
@@ -34,6 +37,7 @@ def F():
     L.fh_print.restype = ctypes.c_int
     L.fh_suite.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.fh_suite.restype = ctypes.c_int
+    _clock_types(L)
 
     def play(fen, moves=""):
         out = ctypes.create_string_buffer(256)
@@ -42,6 +46,12 @@ def F():
         return out.value.decode()
     play.lib = L
     return play
+
+
+def _clock_types(L):
+    L.fh_clock.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double,
+                           ctypes.POINTER(ctypes.c_double)]
+    L.fh_clock.restype = ctypes.c_int
 
 
 START = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
@@ -126,3 +136,87 @@ def test_smp_benchmark_position_sets(F):
         assert L.fh_suite(suite, n, out, 256, ctypes.byref(nm)) == -2
         assert fens[0].rsplit(" ", 2)[0] == first
         assert len(set(fens)) == n
+
+
+CLOCK_KEYS = ("base", "left", "inc", "max_time", "moves_left", "max_moves", "mode", "max_nodes")
+
+
+def clock(L, line="", ply=0, nps=1_000_000, elapsed=-1.0):
+    out = (ctypes.c_double * 8)()
+    rc = L.fh_clock(line.encode(), ply, nps, elapsed, out)
+    assert rc == 0, (line, rc)
+    return dict(zip(CLOCK_KEYS, out))
+
+
+def test_clock_defaults_give_every_move_five_seconds(F, tmp_path):
+    # a fresh engine thinks 5 s per move (src/zeta.c:78-85): fixed time per
+    # move, and after a move the budget is again 5 s, 5 M nodes at 1 M nodes/s
+    # (a fresh copy of the library, so no other test's commands are in it)
+    fresh = tmp_path / "libfenhost_fresh.so"
+    shutil.copy(SO, fresh)
+    L = ctypes.CDLL(str(fresh))
+    _clock_types(L)
+    c = clock(L)
+    assert (c["base"], c["left"], c["inc"], c["max_time"], c["moves_left"], c["mode"]) == (5000, 5000, 0, 5000, 1, 0)
+    c = clock(L, ply=1, elapsed=1234.0)
+    assert (c["left"], c["max_time"], c["max_nodes"]) == (5000, 5000, 5_000_000)
+
+
+def test_clock_conventional_sessions(F):
+    L = F.lib
+    # level 40 5 0 at ply 0: 40 moves in 5 minutes, one move kept spare
+    # (moves left = 40 - (ply+1)/2 % 40 + 1 = 41)
+    c = clock(L, "level 40 5 0")
+    assert (c["base"], c["left"], c["inc"], c["moves_left"], c["max_moves"], c["mode"]) == (300000, 300000, 0, 41, 40, 1)
+    assert c["max_time"] == pytest.approx(300000 / 41) and c["max_nodes"] == 7317073
+    # the base as minutes:seconds
+    c = clock(L, "level 40 2:30 0")
+    assert c["base"] == 150000 and c["max_time"] == pytest.approx(150000 / 41)
+    # the GUI's clock at ply 10: 36 moves left in the session
+    c = clock(L, "time 12000", ply=10)
+    assert (c["left"], c["moves_left"]) == (120000, 36) and c["max_time"] == pytest.approx(120000 / 36)
+    # two moves per minute: after the engine's first move (ply 1) two moves
+    # are left, so the next session's minute is added to the clock
+    clock(L, "level 2 1 0")
+    c = clock(L, ply=1, elapsed=500.0)
+    assert c["moves_left"] == 2 and c["max_time"] == pytest.approx(59500 / 2) and c["left"] == 119500
+
+
+def test_clock_increment_and_sudden_death(F):
+    L = F.lib
+    # level 0 2 12: 2 minutes plus 12 s a move, 40 moves assumed
+    c = clock(L, "level 0 2 12")
+    assert (c["base"], c["inc"], c["moves_left"], c["max_moves"], c["mode"]) == (120000, 12000, 40, 0, 2)
+    assert c["max_time"] == 120000 / 40 + 12000 and c["max_nodes"] == 15_000_000
+    c = clock(L, ply=1, elapsed=2000.0)
+    assert c["left"] == 130000 and c["max_time"] == 130000 / 40 + 12000
+    # level 0 5 0 is the same clock without increment: the time runs down,
+    # it is not reset every move
+    c = clock(L, "level 0 5 0")
+    assert (c["mode"], c["moves_left"], c["max_time"]) == (2, 40, 7500)
+    c = clock(L, ply=1, elapsed=1000.0)
+    assert c["left"] == 299000 and c["max_time"] == 299000 / 40
+    # out of time: no node budget (and no conversion of a negative count)
+    c = clock(L, "time 0")
+    assert (c["left"], c["max_time"], c["max_nodes"]) == (0, 0, 0)
+    c = clock(L, ply=3, elapsed=500.0)
+    assert c["left"] == -500 and c["max_nodes"] == 0
+
+
+def test_clock_fixed_time_per_move(F):
+    L = F.lib
+    # st clears an increment and the session length of an earlier level
+    clock(L, "level 0 2 12")
+    c = clock(L, "st 10")
+    assert (c["base"], c["left"], c["inc"], c["moves_left"], c["max_moves"], c["mode"]) == (10000, 10000, 0, 1, 1, 0)
+    assert c["max_time"] == 10000 and c["max_nodes"] == 10_000_000
+    c = clock(L, ply=5, elapsed=9000.0)
+    assert (c["left"], c["max_time"]) == (10000, 10000)
+
+
+def test_clock_ignores_malformed_levels(F):
+    L = F.lib
+    before = clock(L, "st 7")
+    for line in ("level 40", "level x 5 0", "level -1 5 0"):
+        assert clock(L, line) == before, line
+    assert L.fh_clock(b"otim 100", 0, 1, -1.0, (ctypes.c_double * 8)()) == -1

@@ -60,7 +60,10 @@ bool STM = false;
 int32_t SD = MAXPLY;
 uint64_t MaxNodes = 1;
 int64_t nodes_per_second = 0;
-double MaxTime = 1e12, TimeBase = 0, TimeInc = 0, TimeLeft = 0;
+// time control in milliseconds, 5 s per move until the GUI sets one
+// (src/zeta.c:78-85); timemode 0 = fixed time per move, 1 = conventional
+// clock (MaxMoves per session), 2 = base plus increment
+double MaxTime = 5 * 1000, TimeBase = 5 * 1000, TimeInc = 0, TimeLeft = 5 * 1000;
 int32_t MovesLeft = 1, MaxMoves = 1, timemode = 0;
 uint64_t ABNODECOUNT = 0, TTHITS = 0, TTSCOREHITS = 0, IIDHITS = 0;
 bool xboard_mode = false, xboard_force = false, xboard_post = false, xboard_debug = false;
@@ -430,14 +433,80 @@ void new_game(const char *fen) {
   HashHistory[0] = BOARD[5];
 }
 
+// moves left in the current session of a conventional clock, one move of
+// time kept spare (src/xboard.c:416-417)
+int32_t session_moves_left() { return (MaxMoves - (((PLY + 1) / 2) % MaxMoves)) + 1; }
+
+// the node budget of ms milliseconds at the measured speed (0 once the
+// clock has run out)
+uint64_t nodes_for(double ms) {
+  const double n = ms / 1000 * (double)nodes_per_second;
+  return n > 0 ? (n < 1.8e19 ? (uint64_t)n : UINT64_MAX) : 0;
+}
+
+// the budget of the next move and its node count
+void set_move_budget() {
+  MaxTime = TimeLeft / MovesLeft + TimeInc;
+  MaxNodes = nodes_for(MaxTime);
+}
+
+// the GUI's clock commands (src/xboard.c:384-455):
+//   level MPS BASE INC  BASE in minutes or minutes:seconds, INC in seconds;
+//                       MPS 0 = base plus increment (40 moves assumed),
+//                       else MPS moves per session
+//   st SECONDS          a fixed time per move
+//   time CENTISECONDS   the engine's clock
+// Returns false for any other command.  A level line in neither form leaves
+// the clock as it was (the reference has cleared it by then).
+bool time_command(const char *cmd, const char *line) {
+  if (!strcmp(cmd, "level")) {
+    int mps = 0, mins = 0, secs = 0;
+    double inc = 0;
+    if (sscanf(line, "level %d %d %lf", &mps, &mins, &inc) != 3) {
+      secs = 0;
+      inc = 0;
+      if (sscanf(line, "level %d %d:%d %lf", &mps, &mins, &secs, &inc) != 4) return true;
+    }
+    if (mps < 0) return true;
+    MaxMoves = mps;
+    TimeBase = (60.0 * mins + secs) * 1000;
+    TimeInc = inc * 1000;
+    TimeLeft = TimeBase;
+    timemode = MaxMoves == 0 ? 2 : 1;
+    MovesLeft = timemode == 2 ? 40 : session_moves_left();
+    set_move_budget();
+    return true;
+  }
+  if (!strcmp(cmd, "st")) {
+    double st = 0;
+    sscanf(line, "st %lf", &st);
+    TimeBase = TimeLeft = st * 1000;
+    TimeInc = 0;
+    MovesLeft = MaxMoves = 1;
+    timemode = 0;
+    set_move_budget();
+    return true;
+  }
+  if (!strcmp(cmd, "time")) {
+    double cs = 0;
+    sscanf(line, "time %lf", &cs);
+    TimeLeft = cs * 10;
+    if (timemode == 1) MovesLeft = session_moves_left();
+    set_move_budget();
+    return true;
+  }
+  return false;
+}
+
+// after the engine's move (src/xboard.c:355-371)
 void time_control(double elapsed_ms) {
   TimeLeft -= elapsed_ms;
-  if (timemode == 1) MovesLeft = (MaxMoves - (((PLY + 1) / 2) % MaxMoves)) + 1;
+  if (timemode == 1) MovesLeft = session_moves_left();
   if (timemode == 0) TimeLeft = TimeBase;
   if (timemode == 2) TimeLeft += TimeInc;
   MaxTime = TimeLeft / MovesLeft + TimeInc;
   if (timemode == 1 && MovesLeft == 2) TimeLeft += TimeBase;
-  MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+  MaxNodes = nodes_for(MaxTime);
 }
 
 // think and play a move (src/xboard.c:217-376)
@@ -844,7 +913,7 @@ void xboard() {
       if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
       new_game(STARTFEN);
       SD = MAXPLY;
-      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+      MaxNodes = nodes_for(MaxTime);
       if (!xboard_mode) printboard(BOARD, STM, GAMEPLY, PLY);
       xboard_force = false;
     } else if (!strcmp(cmd, "setboard")) {
@@ -857,35 +926,7 @@ void xboard() {
       think();
     } else if (!strcmp(cmd, "force")) {
       xboard_force = true;
-    } else if (!strcmp(cmd, "level")) {
-      // level MPS BASE INC, BASE as minutes or minutes:seconds (src/xboard.c:384-425)
-      int mps = 0, mins = 0, secs = 0;
-      double inc = 0;
-      if (sscanf(line, "level %d %d:%d %lf", &mps, &mins, &secs, &inc) != 4) {
-        secs = 0;
-        sscanf(line, "level %d %d %lf", &mps, &mins, &inc);
-      }
-      TimeBase = (mins * 60.0 + secs) * 1000;
-      TimeInc = inc * 1000;
-      MaxMoves = mps > 0 ? mps : 1;
-      timemode = mps > 0 ? 1 : (inc > 0 ? 2 : 0);
-      MovesLeft = mps > 0 ? mps : 30;
-      TimeLeft = TimeBase;
-      MaxTime = TimeLeft / MovesLeft + TimeInc;
-      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
-    } else if (!strcmp(cmd, "st")) {
-      double st = 0;
-      sscanf(line, "st %lf", &st);
-      timemode = 0;
-      MovesLeft = 1;
-      TimeBase = TimeLeft = MaxTime = st * 1000;
-      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
-    } else if (!strcmp(cmd, "time")) {
-      double cs = 0;
-      sscanf(line, "time %lf", &cs);
-      TimeLeft = cs * 10;
-      MaxTime = TimeLeft / (MovesLeft > 0 ? MovesLeft : 1) + TimeInc;
-      MaxNodes = (uint64_t)(MaxTime / 1000 * (double)nodes_per_second);
+    } else if (time_command(cmd, line)) {
     } else if (!strcmp(cmd, "otim") || !strcmp(cmd, "memory")) {
     } else if (!strcmp(cmd, "usermove")) {
       char mv[16] = {0};
