@@ -377,6 +377,9 @@ uint32_t rootsearch(const uint64_t *board, bool stm, int32_t depth) {
         }
         out("%i %i %i %" PRIu64 " %s\n", idf, xs, (int32_t)(elapsed * 100), ABNODECOUNT, line.c_str());
       } else {
+        // as the reference (src/search.c:170-171): no thinking line ends the
+        // deepening, so xboard mode without post plays the first completed
+        // iteration's move
         break;
       }
     }
