@@ -157,10 +157,22 @@ def test_think_and_reply_in_a_game():
 
 
 def test_benchsmp_time_to_depth_per_worker_count():
+    # benchsmp (src/xboard.c:822-974): the current position searched to depth
+    # sd with 1, 2, 4, ... workers, then the table of nps and time-to-depth
+    # speedups; the last thinking line of every worker count has the same
+    # depth and score
     p = run("sd 5\nbenchsmp\nquit\n")
     assert p.returncode == 0
-    assert "# overall results" in p.stdout
-    rows = [line for line in p.stdout.splitlines() if line.startswith("### ") and "workers" not in line]
-    assert [int(r.split()[1]) for r in rows] == [1, 2, 4, 8, 16]
-    scores = {line.split("score: ")[1].split()[0] for line in p.stdout.splitlines() if line.startswith("depth: 5,")}
-    assert len(scores) == 1  # the worker count never changes the root score
+    assert "### workers\t#nps\t\t#nps speedup\t#time in s\t#ttd speedup \t#relative ttd speedup ###" in p.stdout
+    table = [line.split() for line in p.stdout.splitlines() if re.match(r"### \d+\t", line)]
+    assert [int(r[1]) for r in table] == [1, 2, 4, 8, 16]
+    assert float(table[0][3]) == 1.0 and float(table[0][5]) == 1.0
+    runs = p.stdout.split("### computing benchsmp depth 5: ###")[1:]
+    assert len(runs) == 5
+    last = []
+    for r in runs:
+        rows = [line.split() for line in r.splitlines()]
+        rows = [x for x in rows if len(x) >= 5 and x[0].isdigit() and x[1].lstrip("-").isdigit()]
+        assert rows, r[:400]
+        last.append((rows[-1][0], rows[-1][1]))
+    assert len(set(last)) == 1 and last[0][0] == "5", last

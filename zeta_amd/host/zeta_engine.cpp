@@ -834,45 +834,57 @@ const char *const HYATT24[24] = {
 // every position is searched to depth sd with 1, 2, 4, ... workers up to the
 // configured count; a table per position, then the time-to-depth speedups
 // over the summed times of all positions.
-void bench_suite(const char *name, const char *const *fens, int npos) {
+// The current position searched by rootsearch to depth SD with 1, 2, 4, ...
+// workers up to the configured count, each count announced by the progress
+// lines of src/xboard.c:846-876, then the table of nps, time to depth and
+// their speedups (src/xboard.c:924-935).  Returns the times; the worker
+// counts in `workers`.
+std::vector<double> smp_runs(const char *name, std::vector<uint64_t> &workers) {
   const uint64_t wmax = totalWorkUnits;
+  std::vector<double> t;
+  std::vector<uint64_t> nps;
+  workers.clear();
+  for (uint64_t w = 1;; w *= 2) {
+    if (w > wmax) w = wmax;
+    out("### doing inits for %s depth %d: ###\n", name, SD);
+    totalWorkUnits = w;
+    if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
+    out("### computing %s depth %d: ###\n", name, SD);
+    out("### work-groups: %" PRIu64 " ###\n", w);
+    const double start = now_ms();
+    rootsearch(BOARD, STM, SD);
+    const double el = (now_ms() - start + 1) / 1000;
+    t.push_back(el);
+    nps.push_back((uint64_t)((double)ABNODECOUNT / el));
+    workers.push_back(w);
+    if (w == wmax) break;
+  }
+  totalWorkUnits = wmax;
+  out("### workers\t#nps\t\t#nps speedup\t#time in s\t#ttd speedup \t#relative ttd speedup ###\n");
+  for (size_t i = 0; i < t.size(); i++) {
+    const double npsx = i ? (double)nps[i] / (double)(nps[0] ? nps[0] : 1) : 1.0;
+    out("### %" PRIu64 "\t\t%" PRIu64 "\t\t%lf\t%lf\t%lf\t%lf \n", workers[i], nps[i], npsx, t[i], t[0] / t[i],
+        i ? t[i - 1] / t[i] : 1.0);
+  }
+  return t;
+}
+
+// benchkaufmann / benchhyatt24 (src/xboard.c:975-1428): smp_runs on every
+// position of the set, then the time-to-depth speedups summed over the set
+void bench_suite(const char *name, const char *const *fens, int npos) {
   std::vector<double> tall;
   std::vector<uint64_t> wall;
   for (int posi = 0; posi < npos; posi++) {
     out("#\n#\n### setting up board %d: ###\n", posi + 1);
     new_game(fens[posi]);
     printboard(BOARD, STM, GAMEPLY, PLY);
-    std::vector<double> t;
-    std::vector<uint64_t> nps, wk;
-    for (uint64_t w = 1;; w *= 2) {
-      if (w > wmax) w = wmax;
-      out("### doing inits for %s depth %d: ###\n", name, SD);
-      totalWorkUnits = w;
-      if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
-      out("### computing %s depth %d: ###\n", name, SD);
-      out("### work-groups: %" PRIu64 " ###\n", w);
-      const double start = now_ms();
-      rootsearch(BOARD, STM, SD);
-      const double el = (now_ms() - start + 1) / 1000;
-      t.push_back(el);
-      nps.push_back((uint64_t)((double)ABNODECOUNT / el));
-      wk.push_back(w);
-      if (w == wmax) break;
-    }
-    out("### workers\t#nps\t\t#nps speedup\t#time in s\t#ttd speedup \t#relative ttd speedup ###\n");
-    for (size_t i = 0; i < t.size(); i++) {
-      const double npsx = i ? (double)nps[i] / (double)(nps[0] ? nps[0] : 1) : 1.0;
-      out("### %" PRIu64 "\t\t%" PRIu64 "\t\t%lf\t%lf\t%lf\t%lf \n", wk[i], nps[i], npsx, t[i], t[0] / t[i],
-          i ? t[i - 1] / t[i] : 1.0);
-    }
+    const std::vector<double> t = smp_runs(name, wall);
     if (tall.size() < t.size()) tall.resize(t.size(), 0.0);
     for (size_t i = 0; i < t.size(); i++) tall[i] += t[i];
-    wall = wk;
   }
   out("#\n# overall results\n### workers\t#ttd speedup\t#rel ttd speedup ###\n");
   for (size_t i = 0; i < tall.size(); i++)
     out("### %" PRIu64 "\t\t%lf\t%lf \n", wall[i], tall[0] / tall[i], i ? tall[i - 1] / tall[i] : 1.0);
-  totalWorkUnits = wmax;
   if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
 }
 
@@ -978,26 +990,10 @@ void xboard() {
           (uint64_t)((double)ABNODECOUNT / el));
       if (!cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
     } else if (!xboard_mode && !strcmp(cmd, "benchsmp")) {
-      // time to depth SD of the current position for 1, 2, 4, .. workers per
-      // root up to the configured count (src/xboard.c:822-974)
-      const uint64_t wmax = totalWorkUnits;
-      std::vector<double> secs;
+      // the current position with 1, 2, 4, ... workers (src/xboard.c:822-974)
       std::vector<uint64_t> ws;
-      for (uint64_t w = 1;; w *= 2) {
-        if (w > wmax) w = wmax;
-        totalWorkUnits = w;
-        if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
-        const double el = benchmark(BOARD, STM, SD);
-        if (el < 0) quitengine(EXIT_FAILURE);
-        secs.push_back(el);
-        ws.push_back(w);
-        if (w == wmax) break;
-      }
-      totalWorkUnits = wmax;
+      smp_runs("benchsmp", ws);
       if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
-      out("#\n# overall results\n### workers\t#ttd speedup\t#rel ttd speedup ###\n");
-      for (size_t i = 0; i < secs.size(); i++)
-        out("### %" PRIu64 "\t\t%lf\t%lf \n", ws[i], secs[0] / secs[i], i ? secs[i - 1] / secs[i] : 1.0);
     } else if (!xboard_mode && !strcmp(cmd, "benchkaufmann")) {
       bench_suite("benchkaufmann", KAUFMANN, 25);
     } else if (!xboard_mode && !strcmp(cmd, "benchhyatt24")) {
