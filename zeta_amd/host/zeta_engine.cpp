@@ -521,15 +521,19 @@ void think() {
     out(STM ? "result 1-0 { resign - max game ply reached }\n" : "result 0-1 { resign - max game ply reached}\n");
     return;
   }
-  const uint32_t move = rootsearch(BOARD, STM, SD);
+  uint32_t move = rootsearch(BOARD, STM, SD);
   const bool none = !(move & 0x3FFFFFFF);
+  const std::vector<uint32_t> legal = legal_moves(BOARD, STM);
   if (ABNODECOUNT == 0 && !none) {
     out(STM ? "result 1-0 { resign - internal error }\n" : "result 0-1 { resign - internal error}\n");
-  } else if (legal_moves(BOARD, STM).empty() && kic) {
+  } else if (legal.empty() && kic) {
     out(STM ? "result 1-0 { checkmate }\n" : "result 0-1 { checkmate }\n");
-  } else if (legal_moves(BOARD, STM).empty()) {
+  } else if (legal.empty()) {
     out("result 1/2-1/2 { stalemate }\n");
   } else {
+    // a search that returned no move while moves exist (none does: the depth-1
+    // iteration always completes) must not put a null move on the board
+    if (none) move = legal[0];
     out("move %s\n", move2can(move).c_str());
     push_move(move);
     if (!xboard_mode || xboard_debug) printboard(BOARD, STM, GAMEPLY, PLY);
