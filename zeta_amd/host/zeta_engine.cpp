@@ -834,10 +834,6 @@ const char *const HYATT24[24] = {
     "3q4/p3b2p/8/1kP5/p2P4/8/1P2Q1PP/6K1 b - -",
     "3q4/p3b2p/8/2P5/pk1P4/3Q4/1P4PP/6K1 b - -"};
 
-// SMP benchmark over a position set (src/xboard.c:975-1201 and 1203-1427):
-// every position is searched to depth sd with 1, 2, 4, ... workers up to the
-// configured count; a table per position, then the time-to-depth speedups
-// over the summed times of all positions.
 // The current position searched by rootsearch to depth SD with 1, 2, 4, ...
 // workers up to the configured count, each count announced by the progress
 // lines of src/xboard.c:846-876, then the table of nps, time to depth and
@@ -873,8 +869,9 @@ std::vector<double> smp_runs(const char *name, std::vector<uint64_t> &workers) {
   return t;
 }
 
-// benchkaufmann / benchhyatt24 (src/xboard.c:975-1428): smp_runs on every
-// position of the set, then the time-to-depth speedups summed over the set
+// SMP benchmark over a position set, benchkaufmann / benchhyatt24
+// (src/xboard.c:975-1201 and 1203-1427): smp_runs on every position of the
+// set, then the time-to-depth speedups over the summed times of all positions
 void bench_suite(const char *name, const char *const *fens, int npos) {
   std::vector<double> tall;
   std::vector<uint64_t> wall;
