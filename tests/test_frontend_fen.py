@@ -88,8 +88,14 @@ def test_roundtrip(F, fen):
      "2kr3r/p1ppqpb1/bn2pnp1/3PN3/1p2P3/2N2Q1p/PPPBBPPP/R3K2R w KQ - 0 2"),
     # a rook captured on its home square takes that side's right with it
     ("r3k2r/8/8/8/8/8/8/R3K2R w KQkq - 0 1", "a1a8", "R3k2r/8/8/8/8/8/8/4K2R b Kk - 0 1"),
-    # promotion is always to a queen
+    # promotion without a letter is to a queen (the engine's own promotions
+    # always are); a letter from the GUI picks the piece (src/io.c:104-113)
     ("8/P7/8/8/8/8/8/k6K w - - 0 1", "a7a8", "Q7/8/8/8/8/8/8/k6K b - - 0 1"),
+    ("8/P7/8/8/8/8/8/k6K w - - 0 1", "a7a8q", "Q7/8/8/8/8/8/8/k6K b - - 0 1"),
+    ("8/P7/8/8/8/8/8/k6K w - - 0 1", "a7a8n", "N7/8/8/8/8/8/8/k6K b - - 0 1"),
+    ("8/P7/8/8/8/8/8/k6K w - - 0 1", "a7a8B", "B7/8/8/8/8/8/8/k6K b - - 0 1"),
+    ("8/P7/8/8/8/8/8/k6K w - - 0 1", "a7a8r", "R7/8/8/8/8/8/8/k6K b - - 0 1"),
+    ("k7/8/8/8/8/8/1p6/R6K b - - 0 1", "b2a1n", "k7/8/8/8/8/8/8/n6K w - - 0 2"),
 ])
 def test_fen_after_moves(F, fen, moves, want):
     assert F(fen, moves) == want

@@ -7,6 +7,7 @@
 // here backed by libzetahip.so.  Position handling (FEN, move text, make /
 // unmake) uses the same chess core as the kernels (zeta_amd/csrc/chess.h,
 // host build).
+#include <ctype.h>
 #include <getopt.h>
 #include <inttypes.h>
 #include <math.h>
@@ -249,12 +250,21 @@ std::string move2can(uint32_t m) {
   return s;
 }
 
-// text -> legal move (the promotion piece is always a queen, Q1); 0 if illegal
+// text -> legal move; 0 if illegal.  The engine's own promotions are to a
+// queen (Q1), but a move the GUI sends may promote to a knight, bishop or
+// rook, and the board must follow it (src/io.c:104-113); no letter or 'q'
+// is a queen
 uint32_t can2move(const char *s, const uint64_t *b, bool stm) {
   if (strlen(s) < 4) return 0;
   const int f = (s[0] - 'a') + 8 * (s[1] - '1'), t = (s[2] - 'a') + 8 * (s[3] - '1');
-  for (uint32_t m : legal_moves(b, stm))
-    if ((int)zc::mv_from(m) == f && (int)zc::mv_to(m) == t) return m;
+  for (uint32_t m : legal_moves(b, stm)) {
+    if ((int)zc::mv_from(m) != f || (int)zc::mv_to(m) != t) continue;
+    const bool promo = ((zc::mv_pfrom(m) >> 1) & 7) == zc::PAWN && ((zc::mv_pto(m) >> 1) & 7) != zc::PAWN;
+    const char c = (char)tolower((unsigned char)s[4]);
+    const int type = c == 'n' ? zc::KNIGHT : c == 'b' ? zc::BISHOP : c == 'r' ? zc::ROOK : 0;
+    if (promo && type) m = (m & ~(0xFu << 22)) | ((uint32_t)((type << 1) | (stm ? 1 : 0)) << 22);
+    return m;
+  }
   return 0;
 }
 
