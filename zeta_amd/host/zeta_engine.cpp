@@ -948,6 +948,12 @@ void xboard() {
       new_game(fen);
       if (!xboard_mode) printboard(BOARD, STM, GAMEPLY, PLY);
     } else if (!strcmp(cmd, "go")) {
+      // only CECP v2 and later (src/xboard.c:217-226)
+      if (xboard_mode && xboard_protover < 2) {
+        out("Error (unsupported protocoll version, < v2): go\n");
+        out("tellusererror (unsupported protocoll version. < v2): go\n");
+        continue;
+      }
       xboard_force = false;
       think();
     } else if (!strcmp(cmd, "force")) {
@@ -955,6 +961,11 @@ void xboard() {
     } else if (time_command(cmd, line)) {
     } else if (!strcmp(cmd, "otim") || !strcmp(cmd, "memory")) {
     } else if (!strcmp(cmd, "usermove")) {
+      // told, but the move is still played (src/xboard.c:464-476)
+      if (xboard_mode && xboard_protover < 2) {
+        out("Error (unsupported protocoll version, < v2): usermove\n");
+        out("tellusererror (unsupported protocoll version, <v2): usermove\n");
+      }
       char mv[16] = {0};
       sscanf(line, "usermove %15s", mv);
       const uint32_t m = can2move(mv, BOARD, STM);
