@@ -935,6 +935,13 @@ void xboard() {
       }
     } else if (!strcmp(cmd, "accepted") || !strcmp(cmd, "rejected")) {
       if (strstr(line, "debug") && !strcmp(cmd, "accepted")) xboard_debug = true;
+      // moves arrive only as usermove: a GUI that refuses it cannot play
+      // this engine (src/xboard.c:106-117)
+      if (strstr(line, "usermove") && !strcmp(cmd, "rejected")) {
+        out("Error (unsupported feature usermove): rejected\n");
+        out("tellusererror (unsupported feature usermove): rejected\n");
+        quitengine(EXIT_FAILURE);
+      }
     } else if (!strcmp(cmd, "new")) {
       if (!cl_release_device() || !cl_init_device((char *)"alphabeta_gpu")) quitengine(EXIT_FAILURE);
       new_game(STARTFEN);
