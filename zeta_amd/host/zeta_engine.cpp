@@ -905,6 +905,8 @@ void print_help() {
   out(" -l, --log          Write output/debug to file zeta.log\n -s, --selftest     Run an internal test\n");
   out(" --guessconfig      Measure the device and write config.txt (quick)\n");
   out(" --guessconfigx     Measure the device and write config.txt (longer)\n");
+  out(" -d 0               Set device ID to 0 for guessconfig (the HIP device; -p is read, unused)\n");
+  out(" -pl, -dl           Say that there are no OpenCL platforms or devices to list\n");
   out(" --config FILE      Read FILE instead of ./config.txt\n");
   out("Commands outside xboard mode: sd N, setboard FEN, go, perft, selftest, bench, benchsmp,\n"
       " benchkaufmann, benchhyatt24 (search depth N with 1, 2, 4, ... workers), help, quit\n");
@@ -1064,23 +1066,40 @@ int main(int argc, char **argv) {
   setvbuf(stdout, nullptr, _IONBF, 0);
   for (int c = 1; c < argc; c++)
     if (!strcmp(argv[c], "-l") || !strcmp(argv[c], "--log")) LogFile = fopen(LOGFILE, "a");
+  // the reference's options (src/zeta.c:529-541), read as it reads them with
+  // getopt_long_only (one or two dashes, unique prefixes: -h, -selftest,
+  // --guessconfigx ...), plus --config FILE; an unknown option prints the help
+  static const struct option opts[] = {
+      {"help", no_argument, nullptr, 0},        {"version", no_argument, nullptr, 0},
+      {"selftest", no_argument, nullptr, 0},    {"log", no_argument, nullptr, 0},
+      {"guessconfig", no_argument, nullptr, 0}, {"guessconfigx", no_argument, nullptr, 0},
+      {"p", required_argument, nullptr, 0},     {"d", required_argument, nullptr, 0},
+      {"pl", no_argument, nullptr, 0},          {"dl", no_argument, nullptr, 0},
+      {"config", required_argument, nullptr, 0}, {nullptr, 0, nullptr, 0}};
   bool want_selftest = false;
-  for (int c = 1; c < argc; c++) {
-    const char *a = argv[c];
-    if (!strcmp(a, "-h") || !strcmp(a, "--help")) {
+  for (int idx = 0, c; (c = getopt_long_only(argc, argv, "", opts, &idx)) != -1;) {
+    if (c != 0) {
       print_help();
-      return EXIT_SUCCESS;
+      return EXIT_FAILURE;
     }
-    if (!strcmp(a, "-v") || !strcmp(a, "--version")) {
-      out("Zeta version: %s\n", VERSION);
-      return EXIT_SUCCESS;
+    switch (idx) {
+      case 0: print_help(); return EXIT_SUCCESS;
+      case 1: out("Zeta version: %s\n", VERSION); return EXIT_SUCCESS;
+      case 2: want_selftest = true; break;
+      case 3: break;  // the log is opened above, before anything is printed
+      case 4:
+      case 5: return guess_config(idx == 5) ? EXIT_SUCCESS : EXIT_FAILURE;
+      case 6: opencl_platform_id = atoi(optarg); break;
+      case 7: opencl_device_id = atoi(optarg); break;
+      case 8:
+      case 9:
+        // src/clquery.c lists OpenCL platforms and devices; this build has HIP
+        // devices, the one -d names for guessconfig
+        out("#> no OpenCL platforms or devices: this build runs on HIP (device %d, -d N)\n", opencl_device_id);
+        return EXIT_SUCCESS;
+      case 10: configfile = optarg; break;
+      default: break;
     }
-    if (!strcmp(a, "-s") || !strcmp(a, "--selftest")) want_selftest = true;
-    if (!strcmp(a, "--guessconfig") || !strcmp(a, "--guessconfigx"))
-      return guess_config(!strcmp(a, "--guessconfigx")) ? EXIT_SUCCESS : EXIT_FAILURE;
-    if ((!strcmp(a, "-d") || !strcmp(a, "--d")) && c + 1 < argc) opencl_device_id = atoi(argv[++c]);
-    if (!strcmp(a, "--config") && c + 1 < argc) configfile = argv[++c];
-    if ((!strcmp(a, "-p") || !strcmp(a, "--p")) && c + 1 < argc) opencl_platform_id = atoi(argv[++c]);
   }
   out("#> Zeta %s\n#> Chess engine, HIP build for MI355X.\n#> engine is initialising...\n", VERSION);
   out("feature done=0\n");
